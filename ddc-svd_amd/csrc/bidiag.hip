@@ -1,0 +1,518 @@
+// bidiag.hip -- Golub-Kahan Householder bidiagonalization on gfx950.
+//
+// Replaces bidiag_par (bidiag_par.c:34-450) and its ten OpenCL kernels.  The
+// result is the reference's: alpha, beta and the unit-norm reflectors
+// (H = I - 2 v v^T, sign(0) = +1, bidiag.c:76-124) stored in place in A.
+//
+// Scheme (m >= n).  Panels of NB columns.  Inside a panel the trailing
+// matrix is kept in its panel-start state A0 and every step uses the
+// low-rank form  M_t = A0 - U Y^T - X V^T  of the partially reduced matrix
+// (u_s / v_s: left / right reflectors of the panel, y_s = 2 M^T u_s,
+// x_s = 2 M v_s).  A step t (global column k) is four stream-ordered kernels:
+//
+//   left_gemv   g = A0[k:m, k+1:n]^T u           2-D grid, partial sums
+//   left_fin    y_t = 2(g - Y (U^T u) - V (X^T u)); row k of M -> r
+//   right_gemv  h = A0[k+1:m, k+1:n] v           2-D grid, partial sums
+//   right_fin   x_t = 2(h - U (Y^T v) - X (V^T v)); column k+1 of M -> c
+//
+// The reflector of the current column/row is never materialised before it
+// is used: each consumer rebuilds u = (c + s|c| e_k)/rl from c and the
+// partial sums of |c|^2 left by the producer (every work-group redoes that
+// tiny fixed-order reduction, so no grid-wide synchronisation is needed and
+// results are bitwise reproducible).  The two GEMVs are the only passes
+// over the trailing matrix: 16 bytes per trailing element per step.  After
+// NB steps the trailing matrix gets A0 -= U Y^T + X V^T on the FP64 matrix
+// cores (gemm_f64_2), one read and one write.
+#include "ddc_common.h"
+
+namespace ddc {
+
+constexpr int NB = 32;        // panel width
+constexpr int RC = 512;       // rows per left_gemv chunk (2 per thread)
+constexpr int CBL = 16;       // columns per left_gemv work-group
+constexpr int RBR = 512;      // rows per right_gemv block (2 per thread)
+constexpr int CCR = 128;      // columns per right_gemv chunk
+constexpr int DPW = 8;        // panel dot products per dot work-group
+constexpr int FIN = 256;      // items per finalize work-group
+constexpr int DSTRIDE = 2 * NB + 1;
+
+struct StepArgs {
+  int m, n, lda, ldy;
+  int k, p, t;
+  double* A;
+  double* alpha;
+  double* beta;
+  double* Y;  // n x NB, ld ldy: y_s
+  double* V;  // n x NB, ld ldy: v_s (contiguous copy of the row reflectors)
+  double* X;  // m x NB, ld lda: x_s
+  double* cvec;  // current column of M (rows k..m-1)
+  double* rvec;  // current row of M (columns k+1..n-1)
+  double* Lpart; // left_gemv partials [chunk][ldy]
+  double* Rpart; // right_gemv partials [chunk][lda]
+  double* Ldot;  // panel dots with c [chunk][2*NB]
+  double* Rdot;  // panel dots with r [chunk][DSTRIDE]
+  double* cnorm; // partial |c|^2 [ncn]
+  double* rnorm; // partial |r|^2 [nrn]
+  int ncn, nrn;
+};
+
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+  return s;
+}
+
+// Sum of cnt partials in a fixed order; every lane of the wave gets it.
+__device__ __forceinline__ double sum_partials(const double* part, int cnt) {
+  double s = 0.0;
+  for (int i = (int)(threadIdx.x & 63); i < cnt; i += 64) s += part[i];
+  return wave_sum(s);
+}
+
+// Householder data of the vector x with |x|^2 = sumsq and first entry x1,
+// as the reference forms it (update_scale_matcol.cl:58-82): diag =
+// -sign(x1)|x|, reflector = (x + sign(x1)|x| e1) / rl with
+// rl = sqrt(2) sqrt(|x|^2 + ||x| x1|).  A zero vector gives rl = 0 and is
+// treated as "no reflection" (the reference divides by zero there).
+struct Refl {
+  double nrm, sgn, rl;
+  __device__ __forceinline__ void make(double sumsq, double x1) {
+    nrm = sqrt(sumsq);
+    sgn = (x1 < 0) ? -1.0 : 1.0;
+    rl = sqrt(2.0) * sqrt(nrm * nrm + fabs(nrm * x1));
+  }
+  __device__ __forceinline__ double diag() const { return rl != 0.0 ? -sgn * nrm : 0.0; }
+  __device__ __forceinline__ double vec(double x, bool first) const {
+    if (rl == 0.0) return 0.0;
+    return (first ? x + sgn * nrm : x) / rl;
+  }
+  // (w^T reflector) from (w^T x) and the first entry w1 of w
+  __device__ __forceinline__ double dot(double wx, double w1) const {
+    return rl != 0.0 ? (wx + sgn * nrm * w1) / rl : 0.0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bd_init(int m, const double* A,
+                                               double* cvec, double* cnorm) {
+  __shared__ double red[4];
+  int row = blockIdx.x * 256 + threadIdx.x;
+  double c = (row < m) ? A[row] : 0.0;
+  if (row < m) cvec[row] = c;
+  double s = block_sum256(c * c, red);
+  if (threadIdx.x == 0) cnorm[blockIdx.x] = s;
+}
+
+// left_gemv: grid.x = ncb column blocks + dot blocks, grid.y = row chunks.
+__global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0) {
+  __shared__ double red[4][CBL];
+  __shared__ double s_scal;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k;
+  if (wave == 0) {
+    double s = sum_partials(a.cnorm, a.ncn);
+    if (lane == 0) s_scal = s;
+  }
+  __syncthreads();
+  Refl R;
+  R.make(s_scal, a.cvec[k]);
+
+  const int r0 = rb0 + blockIdx.y * RC + 2 * tid;
+  const bool inr = r0 < a.lda;
+  double c0 = 0.0, c1 = 0.0;
+  if (inr) {
+    if (r0 >= k && r0 < a.m) c0 = a.cvec[r0];
+    if (r0 + 1 >= k && r0 + 1 < a.m) c1 = a.cvec[r0 + 1];
+  }
+
+  if ((int)blockIdx.x < ncb) {
+    const double u0 = (r0 >= k && r0 < a.m) ? R.vec(c0, r0 == k) : 0.0;
+    const double u1 = (r0 + 1 >= k && r0 + 1 < a.m) ? R.vec(c1, r0 + 1 == k) : 0.0;
+    const int j0 = k + 1 + blockIdx.x * CBL;
+    const int jn = min(CBL, a.n - j0);
+    double acc[CBL];
+#pragma unroll
+    for (int q = 0; q < CBL; ++q) {
+      acc[q] = 0.0;
+      if (q < jn && inr) {
+        double2 v = *reinterpret_cast<const double2*>(a.A + r0 + (int64_t)(j0 + q) * a.lda);
+        acc[q] = v.x * u0 + v.y * u1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CBL; ++q) {
+      double s = wave_sum(acc[q]);
+      if (lane == 0) red[wave][q] = s;
+    }
+    __syncthreads();
+    if (tid < jn)
+      a.Lpart[(int64_t)blockIdx.y * a.ldy + j0 + tid] =
+          red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+    // the reflector itself goes to its final place, column k
+    if (blockIdx.x == 0 && inr) {
+      if (r0 >= k && r0 < a.m) a.A[r0 + (int64_t)k * a.lda] = u0;
+      if (r0 + 1 >= k && r0 + 1 < a.m) a.A[r0 + 1 + (int64_t)k * a.lda] = u1;
+      if (blockIdx.y == 0 && tid == 0) a.alpha[k] = R.diag();
+    }
+  } else {
+    // panel dots with the raw column c: u_s^T c (s < t) and x_s^T c (s < t)
+    const int q0 = (blockIdx.x - ncb) * DPW;
+    const int nd = 2 * a.t;
+    double acc[DPW];
+#pragma unroll
+    for (int q = 0; q < DPW; ++q) {
+      acc[q] = 0.0;
+      int s = q0 + q;
+      if (s < nd && inr) {
+        const double* col = (s < a.t) ? a.A + (int64_t)(a.p + s) * a.lda
+                                      : a.X + (int64_t)(s - a.t) * a.lda;
+        double2 v = *reinterpret_cast<const double2*>(col + r0);
+        acc[q] = v.x * c0 + v.y * c1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < DPW; ++q) {
+      double s = wave_sum(acc[q]);
+      if (lane == 0) red[wave][q] = s;
+    }
+    __syncthreads();
+    if (tid < DPW && q0 + tid < nd)
+      a.Ldot[(int64_t)blockIdx.y * (2 * NB) + q0 + tid] =
+          red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+  }
+}
+
+// left_fin: one thread per trailing column j in [k+1, n).
+__global__ __launch_bounds__(256) void bd_left_fin(StepArgs a, int nchL) {
+  __shared__ double red[4];
+  __shared__ double s_scal;
+  __shared__ double dd[2 * NB];  // u_s^T u, x_s^T u
+  __shared__ double uk[NB + 1];  // u_s[k], s <= t
+  __shared__ double xk[NB];      // x_s[k], s < t
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k, t = a.t;
+  if (wave == 0) {
+    double s = sum_partials(a.cnorm, a.ncn);
+    if (lane == 0) s_scal = s;
+  }
+  __syncthreads();
+  Refl R;
+  R.make(s_scal, a.cvec[k]);
+  if (tid < 2 * t) {
+    double s = 0.0;
+    for (int ch = 0; ch < nchL; ++ch) s += a.Ldot[(int64_t)ch * (2 * NB) + tid];
+    double w1 = (tid < t) ? a.A[k + (int64_t)(a.p + tid) * a.lda]
+                          : a.X[k + (int64_t)(tid - t) * a.lda];
+    dd[tid] = R.dot(s, w1);
+  }
+  if (tid < t) {
+    uk[tid] = a.A[k + (int64_t)(a.p + tid) * a.lda];
+    xk[tid] = a.X[k + (int64_t)tid * a.lda];
+  }
+  if (tid == t) uk[t] = R.vec(a.cvec[k], true);
+  __syncthreads();
+
+  const int j = k + 1 + blockIdx.x * FIN + tid;
+  double rr = 0.0;
+  if (j < a.n) {
+    double g = 0.0;
+    for (int ch = 0; ch < nchL; ++ch) g += a.Lpart[(int64_t)ch * a.ldy + j];
+    double y = g;
+    for (int s = 0; s < t; ++s)
+      y -= a.Y[j + (int64_t)s * a.ldy] * dd[s] + a.V[j + (int64_t)s * a.ldy] * dd[t + s];
+    y *= 2.0;
+    a.Y[j + (int64_t)t * a.ldy] = y;
+    double r = a.A[k + (int64_t)j * a.lda];
+    for (int s = 0; s < t; ++s)
+      r -= uk[s] * a.Y[j + (int64_t)s * a.ldy] + xk[s] * a.V[j + (int64_t)s * a.ldy];
+    r -= uk[t] * y;
+    a.rvec[j] = r;
+    rr = r * r;
+  }
+  double s = block_sum256(rr, red);
+  if (tid == 0) a.rnorm[blockIdx.x] = s;
+}
+
+// right_gemv: grid.x = row blocks + dot blocks, grid.y = column chunks.
+__global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb0) {
+  __shared__ double vs[CCR];
+  __shared__ double red[4][DPW];
+  __shared__ double s_scal;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k;
+  if (wave == 0) {
+    double s = sum_partials(a.rnorm, a.nrn);
+    if (lane == 0) s_scal = s;
+  }
+  __syncthreads();
+  Refl R;
+  R.make(s_scal, a.rvec[k + 1]);
+  const int j0 = k + 1 + blockIdx.y * CCR;
+  const int jn = min(CCR, a.n - j0);
+  if (tid < jn) vs[tid] = R.vec(a.rvec[j0 + tid], j0 + tid == k + 1);
+  __syncthreads();
+
+  if ((int)blockIdx.x < nrb) {
+    const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
+    if (r0 < a.lda) {
+      const double* base = a.A + r0 + (int64_t)j0 * a.lda;
+      double h0 = 0.0, h1 = 0.0;
+      int q = 0;
+      for (; q + 8 <= jn; q += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          v[u] = *reinterpret_cast<const double2*>(base + (int64_t)(q + u) * a.lda);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          h0 += v[u].x * vs[q + u];
+          h1 += v[u].y * vs[q + u];
+        }
+      }
+      for (; q < jn; ++q) {
+        double2 v = *reinterpret_cast<const double2*>(base + (int64_t)q * a.lda);
+        h0 += v.x * vs[q];
+        h1 += v.y * vs[q];
+      }
+      double* out = a.Rpart + (int64_t)blockIdx.y * a.lda;
+      if (r0 >= k + 1 && r0 < a.m) out[r0] = h0;
+      if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
+    }
+    if (blockIdx.x == 0) {
+      __syncthreads();  // row k of this chunk has been read by the loop above
+      if (tid < jn) {
+        a.A[k + (int64_t)(j0 + tid) * a.lda] = vs[tid];
+        a.V[(j0 + tid) + (int64_t)a.t * a.ldy] = vs[tid];
+      }
+      if (blockIdx.y == 0 && tid == 0) a.beta[k] = R.diag();
+    }
+  } else {
+    // panel dots with the raw row r: y_s^T r (s <= t), v_s^T r (s < t)
+    const int q0 = (blockIdx.x - nrb) * DPW;
+    const int nd = 2 * a.t + 1;
+    const int j = j0 + tid;
+    double r = (tid < jn) ? a.rvec[j] : 0.0;
+    double acc[DPW];
+#pragma unroll
+    for (int q = 0; q < DPW; ++q) {
+      acc[q] = 0.0;
+      int s = q0 + q;
+      if (s < nd && tid < jn) {
+        double w = (s <= a.t) ? a.Y[j + (int64_t)s * a.ldy]
+                              : a.V[j + (int64_t)(s - a.t - 1) * a.ldy];
+        acc[q] = w * r;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < DPW; ++q) {
+      double s = wave_sum(acc[q]);
+      if (lane == 0) red[wave][q] = s;
+    }
+    __syncthreads();
+    if (tid < DPW && q0 + tid < nd)
+      a.Rdot[(int64_t)blockIdx.y * DSTRIDE + q0 + tid] =
+          red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+  }
+}
+
+// right_fin: one thread per row in [k+1, m).  noright: the row of M has a
+// single entry, taken as beta with no reflection (bidiag.c:125-129).
+__global__ __launch_bounds__(256) void bd_right_fin(StepArgs a, int nccR, int noright) {
+  __shared__ double red[4];
+  __shared__ double s_scal;
+  __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
+  __shared__ double yk1[NB];       // Y[k+1, s], s <= t
+  __shared__ double vk1[NB];       // V[k+1, s], s <= t
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k, t = a.t;
+  Refl R;
+  R.nrm = R.sgn = R.rl = 0.0;
+  if (!noright) {
+    if (wave == 0) {
+      double s = sum_partials(a.rnorm, a.nrn);
+      if (lane == 0) s_scal = s;
+    }
+    __syncthreads();
+    R.make(s_scal, a.rvec[k + 1]);
+    if (tid < 2 * t + 1) {
+      double s = 0.0;
+      for (int cc = 0; cc < nccR; ++cc) s += a.Rdot[(int64_t)cc * DSTRIDE + tid];
+      double w1 = (tid <= t) ? a.Y[(k + 1) + (int64_t)tid * a.ldy]
+                             : a.V[(k + 1) + (int64_t)(tid - t - 1) * a.ldy];
+      dyv[tid] = R.dot(s, w1);
+    }
+  }
+  if (tid <= t) yk1[tid] = a.Y[(k + 1) + (int64_t)tid * a.ldy];
+  if (tid < t) vk1[tid] = a.V[(k + 1) + (int64_t)tid * a.ldy];
+  if (tid == t) vk1[t] = noright ? 0.0 : R.vec(a.rvec[k + 1], true);
+  __syncthreads();
+
+  const int row = k + 1 + blockIdx.x * FIN + tid;
+  double cc2 = 0.0;
+  if (row < a.m) {
+    double x = 0.0;
+    if (!noright) {
+      double h = 0.0;
+      for (int cc = 0; cc < nccR; ++cc) h += a.Rpart[(int64_t)cc * a.lda + row];
+      x = h;
+      for (int s = 0; s <= t; ++s) x -= a.A[row + (int64_t)(a.p + s) * a.lda] * dyv[s];
+      for (int s = 0; s < t; ++s) x -= a.X[row + (int64_t)s * a.lda] * dyv[t + 1 + s];
+      x *= 2.0;
+    }
+    a.X[row + (int64_t)t * a.lda] = x;
+    if (k + 1 < a.n) {
+      double c = a.A[row + (int64_t)(k + 1) * a.lda];
+      for (int s = 0; s < t; ++s)
+        c -= a.A[row + (int64_t)(a.p + s) * a.lda] * yk1[s] +
+             a.X[row + (int64_t)s * a.lda] * vk1[s];
+      c -= a.A[row + (int64_t)(a.p + t) * a.lda] * yk1[t] + x * vk1[t];
+      a.cvec[row] = c;
+      cc2 = c * c;
+    }
+  }
+  double s = block_sum256(cc2, red);
+  if (tid == 0) a.cnorm[blockIdx.x] = s;
+  if (noright && blockIdx.x == 0 && tid == 0) {
+    a.beta[k] = a.rvec[k + 1];
+    a.A[k + (int64_t)(k + 1) * a.lda] = 0.0;
+    a.V[(k + 1) + (int64_t)t * a.ldy] = 0.0;
+  }
+}
+
+// Last column (m >= n): left reflector only, the 1 x 1 case included
+// (bidiag_par.c:384-397).
+__global__ __launch_bounds__(256) void bd_final_left(StepArgs a) {
+  __shared__ double s_scal;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k;
+  if (wave == 0) {
+    double s = sum_partials(a.cnorm, a.ncn);
+    if (lane == 0) s_scal = s;
+  }
+  __syncthreads();
+  Refl R;
+  R.make(s_scal, a.cvec[k]);
+  const int row = k + blockIdx.x * 256 + tid;
+  if (row < a.m) a.A[row + (int64_t)k * a.lda] = R.vec(a.cvec[row], row == k);
+  if (blockIdx.x == 0 && tid == 0) a.alpha[k] = R.diag();
+}
+
+// ---------------------------------------------------------------------------
+
+struct BidiagBufs {
+  int ldy, nchL_max, nccR_max, ncn_max, nrn_max;
+};
+
+static BidiagBufs bidiag_dims(int m, int n, int lda) {
+  BidiagBufs d;
+  d.ldy = (int)round_up(n, 16);
+  d.nchL_max = (int)ceil_div(lda, RC) + 1;
+  d.nccR_max = (int)ceil_div(n, CCR) + 1;
+  d.ncn_max = (int)ceil_div(m, 256) + 1;
+  d.nrn_max = (int)ceil_div(n, FIN) + 1;
+  return d;
+}
+
+void bidiag_plan(ArenaPlan& p, int m, int n, int lda) {
+  BidiagBufs d = bidiag_dims(m, n, lda);
+  p.add<double>((size_t)d.ldy * NB * 2);   // Y, V
+  p.add<double>((size_t)lda * NB);         // X
+  p.add<double>(lda);                      // cvec
+  p.add<double>(d.ldy);                    // rvec
+  p.add<double>((size_t)d.nchL_max * d.ldy);
+  p.add<double>((size_t)d.nccR_max * lda);
+  p.add<double>((size_t)d.nchL_max * 2 * NB);
+  p.add<double>((size_t)d.nccR_max * DSTRIDE);
+  p.add<double>(d.ncn_max);
+  p.add<double>(d.nrn_max);
+}
+
+int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
+               Arena& ws, hipStream_t st) {
+  if (m < n || n < 1 || lda < m || (lda % 2) != 0) return DDC_E_ARG;
+  BidiagBufs d = bidiag_dims(m, n, lda);
+  StepArgs a;
+  a.m = m;
+  a.n = n;
+  a.lda = lda;
+  a.ldy = d.ldy;
+  a.A = A;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
+  a.V = a.Y + (size_t)d.ldy * NB;
+  a.X = ws.take<double>((size_t)lda * NB);
+  a.cvec = ws.take<double>(lda);
+  a.rvec = ws.take<double>(d.ldy);
+  a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
+  a.Rpart = ws.take<double>((size_t)d.nccR_max * lda);
+  a.Ldot = ws.take<double>((size_t)d.nchL_max * 2 * NB);
+  a.Rdot = ws.take<double>((size_t)d.nccR_max * DSTRIDE);
+  a.cnorm = ws.take<double>(d.ncn_max);
+  a.rnorm = ws.take<double>(d.nrn_max);
+  if (!a.rnorm) return DDC_E_NOMEM;
+
+  DDC_HIP(hipMemsetAsync(a.Y, 0, sizeof(double) * (size_t)d.ldy * NB * 2, st));
+  DDC_HIP(hipMemsetAsync(a.X, 0, sizeof(double) * (size_t)lda * NB, st));
+  DDC_HIP(hipMemsetAsync(beta, 0, sizeof(double) * n, st));
+
+  a.ncn = (int)ceil_div(m, 256);
+  a.nrn = 0;
+  hipLaunchKernelGGL(bd_init, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
+  DDC_LAUNCHED();
+
+  for (int k = 0; k < n; ++k) {
+    a.k = k;
+    a.p = (k / NB) * NB;
+    a.t = k - a.p;
+    if (k == n - 1) {
+      hipLaunchKernelGGL(bd_final_left, dim3((unsigned)ceil_div(m - k, 256)), dim3(256), 0, st, a);
+      DDC_LAUNCHED();
+      break;
+    }
+    // left reflector of column k and the left GEMV
+    const int rb0 = (k / 64) * 64;
+    const int nchL = (int)ceil_div(m - rb0, RC);
+    const int ncb = (int)ceil_div(n - k - 1, CBL);
+    const int ndl = (int)ceil_div(2 * a.t, DPW);
+    hipLaunchKernelGGL(bd_left_gemv, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
+    DDC_LAUNCHED();
+    const int nfl = (int)ceil_div(n - k - 1, FIN);
+    hipLaunchKernelGGL(bd_left_fin, dim3(nfl), dim3(256), 0, st, a, nchL);
+    DDC_LAUNCHED();
+    a.nrn = nfl;
+    // right reflector of row k and the right GEMV
+    const int nfr = (int)ceil_div(m - k - 1, FIN);
+    if (k == n - 2) {
+      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(256), 0, st, a, 0, 1);
+      DDC_LAUNCHED();
+    } else {
+      const int rbr = ((k + 1) / 64) * 64;
+      const int nrb = (int)ceil_div(m - rbr, RBR);
+      const int ncc = (int)ceil_div(n - k - 1, CCR);
+      const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
+      hipLaunchKernelGGL(bd_right_gemv, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
+      DDC_LAUNCHED();
+      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(256), 0, st, a, ncc, 0);
+      DDC_LAUNCHED();
+    }
+    a.ncn = nfr;
+    // blocked trailing update at the end of a full panel
+    if (a.t == NB - 1 && a.p + NB < n) {
+      const int r0 = a.p + NB;
+      GemmOperand U1{A + r0 + (int64_t)a.p * lda, lda, 'N', MASK_NONE, 0};
+      GemmOperand Y1{a.Y + r0, d.ldy, 'T', MASK_NONE, 0};
+      GemmOperand X2{a.X + r0, lda, 'N', MASK_NONE, 0};
+      GemmOperand V2{a.V + r0, d.ldy, 'T', MASK_NONE, 0};
+      DDC_TRY(gemm_f64_2(m - r0, n - r0, -1.0, U1, Y1, NB, X2, V2, NB, 1.0,
+                         A + r0 + (int64_t)r0 * lda, lda, st));
+    }
+  }
+  return DDC_OK;
+}
+
+}  // namespace ddc

@@ -1,0 +1,432 @@
+// capi.hip -- the C-ABI of libddcsvd (declared in include/svd_gpu.h).
+//
+// One process-wide context: the current HIP device at first use (so
+// HIP_VISIBLE_DEVICES selects it; the reference asks on stdin instead,
+// bidiag_par.c:151), one stream, and one grow-only device arena that every
+// call carves its buffers from.  Calls are serialised by a mutex.
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/svd_gpu.h"
+#include "ddc_common.h"
+
+namespace ddc {
+
+namespace {
+std::mutex g_mu;
+char g_err[512] = "";
+double g_times[6] = {0, 0, 0, 0, 0, 0};
+struct Ctx {
+  bool init = false;
+  int dev = 0;
+  hipStream_t st = nullptr;
+  char* arena = nullptr;
+  size_t cap = 0;
+} g_ctx;
+
+void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+}  // namespace
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_err("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  return DDC_E_HIP;
+}
+
+static int ctx_init() {
+  if (g_ctx.init) return DDC_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_err("no HIP device available");
+    return DDC_E_NODEV;
+  }
+  DDC_HIP(hipGetDevice(&g_ctx.dev));
+  DDC_HIP(hipStreamCreateWithFlags(&g_ctx.st, hipStreamNonBlocking));
+  g_ctx.init = true;
+  return DDC_OK;
+}
+
+static int arena_get(size_t bytes, Arena& ws) {
+  if (bytes > g_ctx.cap) {
+    if (g_ctx.arena) DDC_HIP(hipFree(g_ctx.arena));
+    g_ctx.arena = nullptr;
+    g_ctx.cap = 0;
+    if (hipMalloc(&g_ctx.arena, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_err("device allocation of %zu bytes failed", bytes);
+      return DDC_E_NOMEM;
+    }
+    g_ctx.cap = bytes;
+  }
+  ws.base = g_ctx.arena;
+  ws.cap = g_ctx.cap;
+  ws.used = 0;
+  return DDC_OK;
+}
+
+// flag[0] = 1 if any entry of the m x n matrix is not finite
+__global__ void k_check_finite(int m, int n, const double* A, int lda, int* flag) {
+  int64_t total = (int64_t)m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = e / m, i = e - j * m;
+    double v = A[i + j * lda];
+    if (!isfinite(v)) flag[0] = 1;
+  }
+}
+
+// Workspace of the tall (m >= n) pipeline.
+struct TallBufs {
+  int lda_p, ldb;
+  double *Ap, *alpha, *beta, *sigma, *Vt, *Ut, *U, *V;
+  int* flag;
+};
+
+static void tall_plan(ArenaPlan& p, int m, int n, bool want_uv_out) {
+  int lda_p = (int)round_up(m, 16);
+  int ldb = (int)round_up(n, 16);
+  p.add<double>((size_t)lda_p * n);
+  p.add<double>(n);
+  p.add<double>(n);
+  p.add<double>(n);
+  p.add<double>((size_t)ldb * n * 2);
+  if (want_uv_out) {
+    p.add<double>((size_t)m * n);
+    p.add<double>((size_t)n * n);
+  }
+  p.add<int>(64);
+  bidiag_plan(p, m, n, lda_p);
+  dc_plan(p, n);
+  twisted_plan(p, n);
+  backtrans_plan(p, m, n);
+}
+
+static int tall_take(Arena& ws, int m, int n, bool want_uv_out, TallBufs& b) {
+  b.lda_p = (int)round_up(m, 16);
+  b.ldb = (int)round_up(n, 16);
+  b.Ap = ws.take<double>((size_t)b.lda_p * n);
+  b.alpha = ws.take<double>(n);
+  b.beta = ws.take<double>(n);
+  b.sigma = ws.take<double>(n);
+  b.Vt = ws.take<double>((size_t)b.ldb * n * 2);
+  b.Ut = b.Vt ? b.Vt + (size_t)b.ldb * n : nullptr;
+  b.U = b.V = nullptr;
+  if (want_uv_out) {
+    b.U = ws.take<double>((size_t)m * n);
+    b.V = ws.take<double>((size_t)n * n);
+  }
+  b.flag = ws.take<int>(64);
+  return b.flag ? DDC_OK : DDC_E_NOMEM;
+}
+
+struct Timer {
+  hipEvent_t ev[7];
+  int n = 0;
+  hipStream_t st;
+  int create(hipStream_t s) {
+    st = s;
+    for (int i = 0; i < 7; ++i) DDC_HIP(hipEventCreate(&ev[i]));
+    return DDC_OK;
+  }
+  int mark() {
+    if (n < 7) DDC_HIP(hipEventRecord(ev[n++], st));
+    return DDC_OK;
+  }
+  void finish() {
+    for (int i = 0; i < 6; ++i) g_times[i] = 0.0;
+    for (int i = 0; i + 1 < n && i < 6; ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) g_times[i] = ms;
+    }
+  }
+  ~Timer() {
+    for (int i = 0; i < 7; ++i) (void)hipEventDestroy(ev[i]);
+  }
+};
+
+// Runs the four stages on b.Ap (already holding A with zero padding).
+// Results: sigma in b.sigma, U (m x n) in dU/ldu, V (n x n) in dV/ldv.
+static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu,
+                       double* dV, int ldv, hipStream_t st, Timer& tm) {
+  DDC_HIP(hipMemsetAsync(b.flag, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p, b.flag);
+  DDC_LAUNCHED();
+  int hflag = 0;
+  DDC_HIP(hipMemcpyAsync(&hflag, b.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  if (hflag) {
+    set_err("input matrix holds NaN or Inf");
+    return DDC_E_NUMERIC;
+  }
+  DDC_TRY(tm.mark());
+  DDC_TRY(bidiag_run(m, n, b.Ap, b.lda_p, b.alpha, b.beta, ws, st));
+  DDC_TRY(tm.mark());
+  DDC_TRY(dc_run(n, b.alpha, b.beta, b.sigma, ws, st));
+  DDC_TRY(tm.mark());
+  DDC_TRY(twisted_run(n, b.alpha, b.beta, b.sigma, b.Vt, b.Ut, b.ldb, ws, st));
+  DDC_TRY(tm.mark());
+  DDC_TRY(backtrans_run(m, n, b.Ap, b.lda_p, b.Ut, b.Vt, b.ldb, dU, ldu, dV, ldv, ws, st));
+  DDC_TRY(tm.mark());
+  return DDC_OK;
+}
+
+static int check_dims(int m, int n) {
+  if (m < 1 || n < 1) {
+    set_err("invalid dimensions m=%d n=%d", m, n);
+    return DDC_E_ARG;
+  }
+  if ((int64_t)m * n > ((int64_t)1 << 40)) {
+    set_err("matrix too large");
+    return DDC_E_ARG;
+  }
+  return DDC_OK;
+}
+
+// host buffers, the drop-in semantics
+static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V) {
+  DDC_TRY(check_dims(m, n));
+  if (!A || !sigma || !U || !V) {
+    set_err("null pointer argument");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = g_ctx.st;
+  const bool wide = m < n;
+  const int mt = wide ? n : m, nt = wide ? m : n;  // tall problem dims
+  ArenaPlan plan;
+  tall_plan(plan, mt, nt, true);
+  if (wide) plan.add<double>((size_t)m * n);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  TallBufs b;
+  DDC_TRY(tall_take(ws, mt, nt, true, b));
+  double* Araw = wide ? ws.take<double>((size_t)m * n) : nullptr;
+  Timer tm;
+  DDC_TRY(tm.create(st));
+  DDC_TRY(tm.mark());
+  DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * nt, st));
+  if (!wide) {
+    DDC_HIP(hipMemcpy2DAsync(b.Ap, sizeof(double) * b.lda_p, A, sizeof(double) * m,
+                             sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+  } else {
+    DDC_HIP(hipMemcpyAsync(Araw, A, sizeof(double) * (size_t)m * n, hipMemcpyHostToDevice, st));
+    DDC_TRY(transpose_f64(m, n, Araw, m, b.Ap, b.lda_p, st));
+  }
+  DDC_TRY(tall_stages(mt, nt, b, ws, b.U, mt, b.V, nt, st, tm));
+  // outputs
+  DDC_HIP(hipMemcpyAsync(sigma, b.sigma, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
+  if (!wide) {
+    DDC_HIP(hipMemcpyAsync(U, b.U, sizeof(double) * (size_t)m * n, hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipMemcpyAsync(V, b.V, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, st));
+    // A is left holding the reflectors, as bidiag_par leaves it (bidiag_par.c:415-418)
+    DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, b.Ap, sizeof(double) * b.lda_p,
+                             sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
+  } else {
+    // A^T = U' S V'^T  =>  A = V' S U'^T : U = V' (m x m), V = U' (n x m)
+    DDC_HIP(hipMemcpyAsync(U, b.V, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipMemcpyAsync(V, b.U, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToHost, st));
+  }
+  DDC_TRY(tm.mark());
+  DDC_HIP(hipStreamSynchronize(st));
+  tm.finish();
+  return DDC_OK;
+}
+
+static int svd_dev(int m, int n, const double* dA, int lda, double* dsigma, double* dU,
+                   int ldu, double* dV, int ldv, hipStream_t user) {
+  DDC_TRY(check_dims(m, n));
+  const int mn = m < n ? m : n;
+  if (!dA || !dsigma || !dU || !dV || lda < m || ldu < m || ldv < n) {
+    set_err("invalid device arguments");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = user ? user : g_ctx.st;
+  const bool wide = m < n;
+  const int mt = wide ? n : m, nt = mn;
+  ArenaPlan plan;
+  tall_plan(plan, mt, nt, wide);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  TallBufs b;
+  DDC_TRY(tall_take(ws, mt, nt, wide, b));
+  Timer tm;
+  DDC_TRY(tm.create(st));
+  DDC_TRY(tm.mark());
+  DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * nt, st));
+  if (!wide) {
+    DDC_HIP(hipMemcpy2DAsync(b.Ap, sizeof(double) * b.lda_p, dA, sizeof(double) * lda,
+                             sizeof(double) * m, n, hipMemcpyDeviceToDevice, st));
+    DDC_TRY(tall_stages(mt, nt, b, ws, dU, ldu, dV, ldv, st, tm));
+  } else {
+    DDC_TRY(transpose_f64(m, n, dA, lda, b.Ap, b.lda_p, st));
+    DDC_TRY(tall_stages(mt, nt, b, ws, b.U, mt, b.V, nt, st, tm));
+    DDC_HIP(hipMemcpy2DAsync(dU, sizeof(double) * ldu, b.V, sizeof(double) * nt,
+                             sizeof(double) * m, m, hipMemcpyDeviceToDevice, st));
+    DDC_HIP(hipMemcpy2DAsync(dV, sizeof(double) * ldv, b.U, sizeof(double) * mt,
+                             sizeof(double) * n, m, hipMemcpyDeviceToDevice, st));
+  }
+  DDC_HIP(hipMemcpyAsync(dsigma, b.sigma, sizeof(double) * nt, hipMemcpyDeviceToDevice, st));
+  DDC_TRY(tm.mark());
+  DDC_HIP(hipStreamSynchronize(st));
+  tm.finish();
+  return DDC_OK;
+}
+
+static int bidiag_host(int m, int n, double* A, double* alpha, double* beta) {
+  DDC_TRY(check_dims(m, n));
+  if (m < n) {
+    set_err("bidiag_gpu requires m >= n");
+    return DDC_E_ARG;
+  }
+  if (!A || !alpha || (n > 1 && !beta)) {
+    set_err("null pointer argument");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = g_ctx.st;
+  const int lda_p = (int)round_up(m, 16);
+  ArenaPlan plan;
+  plan.add<double>((size_t)lda_p * n);
+  plan.add<double>(n);
+  plan.add<double>(n);
+  bidiag_plan(plan, m, n, lda_p);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  double* Ap = ws.take<double>((size_t)lda_p * n);
+  double* da = ws.take<double>(n);
+  double* db = ws.take<double>(n);
+  DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
+  DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, A, sizeof(double) * m,
+                           sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+  DDC_TRY(bidiag_run(m, n, Ap, lda_p, da, db, ws, st));
+  DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, Ap, sizeof(double) * lda_p,
+                           sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipMemcpyAsync(alpha, da, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  if (n > 1)
+    DDC_HIP(hipMemcpyAsync(beta, db, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  return DDC_OK;
+}
+
+static int bdsvd_host(int n, const double* alpha, const double* beta, double* sigma,
+                      int refine, double* X, double* Y) {
+  if (n < 1 || !alpha || !sigma || (n > 1 && !beta)) {
+    set_err("invalid arguments");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = g_ctx.st;
+  const int ldb = (int)round_up(n, 16);
+  const bool vec = refine || X;
+  ArenaPlan plan;
+  plan.add<double>(n);
+  plan.add<double>(n);
+  plan.add<double>(n);
+  dc_plan(plan, n);
+  if (vec) {
+    plan.add<double>((size_t)ldb * n * 2);
+    plan.add<double>((size_t)n * n);
+    twisted_plan(plan, n);
+  }
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  double* da = ws.take<double>(n);
+  double* db = ws.take<double>(n);
+  double* ds = ws.take<double>(n);
+  DDC_HIP(hipMemcpyAsync(da, alpha, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  DDC_HIP(hipMemsetAsync(db, 0, sizeof(double) * n, st));
+  if (n > 1)
+    DDC_HIP(hipMemcpyAsync(db, beta, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+  DDC_TRY(dc_run(n, da, db, ds, ws, st));
+  if (vec) {
+    double* Vt = ws.take<double>((size_t)ldb * n * 2);
+    double* Ut = Vt + (size_t)ldb * n;
+    double* tmp = ws.take<double>((size_t)n * n);
+    DDC_TRY(twisted_run(n, da, db, ds, Vt, Ut, ldb, ws, st));
+    if (X) {
+      DDC_TRY(transpose_f64(n, n, Vt, ldb, tmp, n, st));
+      DDC_HIP(hipMemcpyAsync(X, tmp, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, st));
+      DDC_HIP(hipStreamSynchronize(st));
+      DDC_TRY(transpose_f64(n, n, Ut, ldb, tmp, n, st));
+      DDC_HIP(hipMemcpyAsync(Y, tmp, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, st));
+    }
+  }
+  DDC_HIP(hipMemcpyAsync(sigma, ds, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  return DDC_OK;
+}
+
+}  // namespace ddc
+
+using namespace ddc;
+
+extern "C" {
+
+int svd_gpu_status(int m, int n, double* A, double* sigma, double* U, double* V) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return svd_host(m, n, A, sigma, U, V);
+}
+
+void svd_gpu(int m, int n, double* A, double* sigma, double* U, double* V) {
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_err[0] = 0;
+    rc = svd_host(m, n, A, sigma, U, V);
+  }
+  if (rc != 0) {
+    fprintf(stderr, "svd_gpu: %s (status %d)\n", g_err, rc);
+    abort();
+  }
+}
+
+int svd_gpu_dev(int m, int n, const double* dA, int lda, double* dsigma, double* dU,
+                int ldu, double* dV, int ldv, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return svd_dev(m, n, dA, lda, dsigma, dU, ldu, dV, ldv, (hipStream_t)stream);
+}
+
+int bidiag_gpu(int m, int n, double* A, double* alpha, double* beta) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return bidiag_host(m, n, A, alpha, beta);
+}
+
+int bdsvd_values_gpu(int n, const double* alpha, const double* beta, double* sigma,
+                     int refine) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return bdsvd_host(n, alpha, beta, sigma, refine, nullptr, nullptr);
+}
+
+int bdsvd_vectors_gpu(int n, const double* alpha, const double* beta, double* sigma,
+                      double* X, double* Y) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  if (!X || !Y) {
+    set_err("null pointer argument");
+    return DDC_E_ARG;
+  }
+  return bdsvd_host(n, alpha, beta, sigma, 1, X, Y);
+}
+
+int svd_gpu_last_timings(double* ms, int count) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int k = count < 6 ? count : 6;
+  for (int i = 0; i < k; ++i) ms[i] = g_times[i];
+  return k;
+}
+
+const char* svd_gpu_last_error(void) { return g_err; }
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// ddc_common.h -- shared internals of libddcsvd (gfx950, HIP).
+//
+// Error model: every internal routine returns an int status (0 = success,
+// negative = DDC_E_* code).  The reference-compatible svd_gpu() turns a
+// failure into a message on stderr and abort(), as the reference does
+// (cl-helper.h:47-55, svd_gpu.c:85-96); svd_gpu_status() returns the code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+#define DDC_OK 0
+#define DDC_E_ARG -1
+#define DDC_E_NOMEM -2
+#define DDC_E_HIP -3
+#define DDC_E_NODEV -4
+#define DDC_E_NUMERIC -5
+
+namespace ddc {
+
+// Records the first HIP failure of the calling thread so entry points can
+// report it; returns DDC_E_HIP on failure.
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define DDC_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return ::ddc::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define DDC_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != 0) return rc_;  \
+  } while (0)
+
+// Kernel launches are checked through hipGetLastError after the launch.
+#define DDC_LAUNCHED() DDC_HIP(hipGetLastError())
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+// A bump allocator over one device allocation; every stage asks it for the
+// buffers it needs, so one svd_gpu call makes a single hipMalloc.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  template <typename T>
+  T* take(size_t count) {
+    size_t bytes = round_up((int64_t)(count * sizeof(T)), 256);
+    if (used + bytes > cap) return nullptr;
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+// Bytes an Arena needs for take<T>(count) calls, to size the allocation.
+struct ArenaPlan {
+  size_t bytes = 0;
+  template <typename T>
+  void add(size_t count) { bytes += round_up((int64_t)(count * sizeof(T)), 256); }
+};
+
+// ---------------------------------------------------------------------------
+// Stage entry points (device pointers, stream-ordered).  Each has a
+// *_plan() that adds its workspace to an ArenaPlan.
+// ---------------------------------------------------------------------------
+
+// Golub-Kahan bidiagonalization of the m x n (m >= n) column-major matrix A
+// (leading dimension lda, rows m..lda-1 must hold zeros) in place: reflectors
+// as the reference stores them (bidiag_par.c:34-450), alpha[n], beta[n] with
+// beta[n-1] = 0.  nb is the panel width of the blocked update.
+void bidiag_plan(ArenaPlan& p, int m, int n, int lda);
+int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
+               Arena& ws, hipStream_t st);
+
+// Singular values (ascending) of the n x (n+1) upper bidiagonal with diagonal
+// a[n] and super-diagonal b[n] by divide and conquer.
+void dc_plan(ArenaPlan& p, int n);
+int dc_run(int n, const double* a, const double* b, double* sigma, Arena& ws,
+           hipStream_t st);
+
+// Singular vectors of the n x n upper bidiagonal (a[n], b[n-1]) for the
+// ascending singular values sigma (refined in place to high relative
+// accuracy).  Vectors are written sigma-minor: Vt[j*ldv + i] = v_i[j] and
+// Ut[j*ldv + i] = u_i[j], so B v_i = sigma_i u_i.
+void twisted_plan(ArenaPlan& p, int n);
+int twisted_run(int n, const double* a, const double* b, double* sigma,
+                double* Vt, double* Ut, int ldv, Arena& ws, hipStream_t st);
+
+// U = Q_left [Yb; 0] (m x n) and V = Q_right Xb (n x n) from the reflectors
+// stored in A (m x n, lda), where Yb/Xb are given sigma-minor as from
+// twisted_run (ld ldb).  U gets leading dimension ldu, V ldv.
+void backtrans_plan(ArenaPlan& p, int m, int n);
+int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
+                  const double* Vt, int ldb, double* U, int ldu, double* V,
+                  int ldv, Arena& ws, hipStream_t st);
+
+// D (c x r) = S^T for the r x c column-major matrix S.
+int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd,
+                  hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA GEMM: C[M x N] = beta*C + alpha * op(A) * op(B), column-major.
+// op: 'N' or 'T'.  The mask selects a structured operand; it applies to the
+// STORED element X[i + j*ld], before op:
+//   MASK_NONE:   plain
+//   MASK_LOWER:  X(i, j) = 0 unless i >= j + moff  (left reflector panel)
+//   MASK_UPPER1: X(i, j) = 0 unless j >= i + moff  (right reflectors, rows)
+// ---------------------------------------------------------------------------
+enum { MASK_NONE = 0, MASK_LOWER = 1, MASK_UPPER1 = 2 };
+struct GemmOperand {
+  const double* p;
+  int ld;
+  char op;     // 'N' or 'T'
+  int mask;    // MASK_*
+  int moff;    // mask offset
+};
+int gemm_f64(int M, int N, int K, double alpha, GemmOperand A, GemmOperand B,
+             double beta, double* C, int ldc, hipStream_t st);
+// Two operand pairs accumulated into one pass over C.
+int gemm_f64_2(int M, int N, double alpha, GemmOperand A1, GemmOperand B1,
+               int K1, GemmOperand A2, GemmOperand B2, int K2, double beta,
+               double* C, int ldc, hipStream_t st);
+
+// Deterministic wave64 sum (fixed butterfly order).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace ddc

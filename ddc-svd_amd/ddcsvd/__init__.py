@@ -1,0 +1,170 @@
+"""Python access to libddcsvd (the MI355X svd_gpu path) through its C-ABI.
+
+The functions mirror the reference's C entry points (svd_gpu.h:5,
+bidiag_par.h:30, Calculations-Parallel.h:41, parallel-twisted.h:18-19):
+column-major float64 arrays, singular values ascending, column i of U and V
+paired with sigma[i].  Every call goes to the HIP library; there is no CPU
+fallback, and a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libddcsvd.so")
+HEADER = os.path.join(ROOT, "include", "svd_gpu.h")
+
+OK, E_ARG, E_NOMEM, E_HIP, E_NODEV, E_NUMERIC = 0, -1, -2, -3, -4, -5
+
+
+class SvdGpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libddcsvd status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def declared_symbols() -> list[str]:
+    """Names of the functions include/svd_gpu.h declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", text)) - {"if", "defined"})
+
+
+def load():
+    """Load the built library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python ddc-svd_amd/build.py`")
+        lib = ctypes.CDLL(LIB_PATH)
+        i, d, v = ctypes.c_int, _dp, ctypes.c_void_p
+        lib.svd_gpu.argtypes = [i, i, d, d, d, d]
+        lib.svd_gpu.restype = None
+        lib.svd_gpu_status.argtypes = [i, i, d, d, d, d]
+        lib.svd_gpu_status.restype = i
+        lib.svd_gpu_dev.argtypes = [i, i, v, i, v, v, i, v, i, v]
+        lib.svd_gpu_dev.restype = i
+        lib.bidiag_gpu.argtypes = [i, i, d, d, d]
+        lib.bidiag_gpu.restype = i
+        lib.bdsvd_values_gpu.argtypes = [i, d, d, d, i]
+        lib.bdsvd_values_gpu.restype = i
+        lib.bdsvd_vectors_gpu.argtypes = [i, d, d, d, d, d]
+        lib.bdsvd_vectors_gpu.restype = i
+        lib.svd_gpu_last_timings.argtypes = [d, i]
+        lib.svd_gpu_last_timings.restype = i
+        lib.svd_gpu_last_error.argtypes = []
+        lib.svd_gpu_last_error.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc: int):
+    if rc != OK:
+        raise SvdGpuError(rc, load().svd_gpu_last_error().decode())
+
+
+def last_error() -> str:
+    return load().svd_gpu_last_error().decode()
+
+
+def last_timings() -> dict:
+    """Stage times (ms) of the last svd call."""
+    out = np.zeros(6)
+    load().svd_gpu_last_timings(_p(out), 6)
+    return dict(zip(["copy_in", "bidiag", "values", "vectors", "backtransform", "copy_out"],
+                    out.tolist()))
+
+
+def svd(A: np.ndarray, raw: bool = False):
+    """svd_gpu on host arrays.  Returns (sigma, U[:, :mn], V[:, :mn], A_after).
+
+    raw=True calls the int-returning svd_gpu_status and returns the status
+    instead of raising."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    mn = min(m, n)
+    sigma = np.zeros(mn)
+    U = np.zeros((m, m), order="F")
+    V = np.zeros((n, n), order="F")
+    rc = load().svd_gpu_status(m, n, _p(A), _p(sigma), _p(U), _p(V))
+    if raw:
+        return rc, sigma, U, V, A
+    _check(rc)
+    return sigma, U[:, :mn].copy(order="F"), V[:, :mn].copy(order="F"), A
+
+
+def svd_dev(A, sigma, U, V, stream=None) -> None:
+    """svd_gpu_dev on torch.float64 CUDA tensors (column-major views: pass
+    A.t().contiguous().t()-style tensors or use `svd_torch`)."""
+    m, n = A.shape
+    lda = A.stride(1)
+    rc = load().svd_gpu_dev(m, n, ctypes.c_void_p(A.data_ptr()), lda,
+                            ctypes.c_void_p(sigma.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                            U.stride(1), ctypes.c_void_p(V.data_ptr()), V.stride(1),
+                            ctypes.c_void_p(stream))
+    _check(rc)
+
+
+def col_major(t):
+    """A column-major (Fortran-ordered) copy of a 2-D torch tensor."""
+    return t.t().contiguous().t()
+
+
+def svd_torch(A):
+    """Device SVD of a CUDA float64 tensor; returns (sigma, U, V) tensors."""
+    import torch
+    A = col_major(A)
+    m, n = A.shape
+    mn = min(m, n)
+    sigma = torch.empty(mn, dtype=torch.float64, device=A.device)
+    U = col_major(torch.empty((m, mn), dtype=torch.float64, device=A.device))
+    V = col_major(torch.empty((n, mn), dtype=torch.float64, device=A.device))
+    stream = torch.cuda.current_stream(A.device).cuda_stream
+    svd_dev(A, sigma, U, V, stream)
+    return sigma, U, V
+
+
+def bidiag(A: np.ndarray):
+    """bidiag_gpu: returns (A_with_reflectors, alpha, beta)."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m, n = A.shape
+    alpha = np.zeros(n)
+    beta = np.zeros(max(n - 1, 1))
+    _check(load().bidiag_gpu(m, n, _p(A), _p(alpha), _p(beta)))
+    return A, alpha, beta[: n - 1]
+
+
+def bdsvd_values(alpha, beta, refine: bool = False) -> np.ndarray:
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    n = len(alpha)
+    beta = np.ascontiguousarray(np.append(np.asarray(beta, dtype=np.float64)[: n - 1], 0.0))
+    sigma = np.zeros(n)
+    _check(load().bdsvd_values_gpu(n, _p(alpha), _p(beta), _p(sigma), 1 if refine else 0))
+    return sigma
+
+
+def bdsvd_vectors(alpha, beta):
+    """Returns (sigma, X, Y): B X[:, i] = sigma_i Y[:, i]."""
+    alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+    n = len(alpha)
+    beta = np.ascontiguousarray(np.append(np.asarray(beta, dtype=np.float64)[: n - 1], 0.0))
+    sigma = np.zeros(n)
+    X = np.zeros((n, n), order="F")
+    Y = np.zeros((n, n), order="F")
+    _check(load().bdsvd_vectors_gpu(n, _p(alpha), _p(beta), _p(sigma), _p(X), _p(Y)))
+    return sigma, X, Y

@@ -1,0 +1,102 @@
+/*
+ * svd_gpu.h -- C interface of libddcsvd, the MI355X (gfx950) implementation
+ * of the ddc-svd SVD path.
+ *
+ * Link with -lddcsvd (plus the HIP runtime, which the library pulls in).
+ * All matrices are column-major doubles.  Singular values come out in
+ * ascending order and column i of U and V pairs with sigma[i], as in the
+ * reference (Calculations-Parallel.c:56, svd_gpu.c:117-121).
+ */
+#ifndef DDCSVD_SVD_GPU_H
+#define DDCSVD_SVD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes of the int-returning entry points. */
+#define SVD_GPU_OK 0
+#define SVD_GPU_E_ARG -1     /* invalid dimensions or pointers */
+#define SVD_GPU_E_NOMEM -2   /* device or host allocation failed */
+#define SVD_GPU_E_HIP -3     /* a HIP runtime call failed */
+#define SVD_GPU_E_NODEV -4   /* no usable GPU */
+#define SVD_GPU_E_NUMERIC -5 /* non-finite input */
+
+/*
+ * Drop-in replacement of the reference entry point
+ *   void svd_gpu(int m, int n, double* A, double* sigma, double* U, double* V)
+ * declared at svd_gpu.h:5 and defined at svd_gpu.c:53-131.
+ *   A      m x n host buffer; overwritten (as in the reference, with the
+ *          Householder reflectors of the bidiagonalization for m >= n)
+ *   sigma  min(m,n) singular values, ascending
+ *   U      m x m host buffer; the first min(m,n) columns are written
+ *   V      n x n host buffer; the first min(m,n) columns are written
+ *          (V itself, not V^T)
+ * Errors print a message to stderr and abort(), as the reference does
+ * (cl-helper.h:47-55, svd_gpu.c:85-96).  Thread-safe (calls serialise).
+ */
+void svd_gpu(int m, int n, double *A, double *sigma, double *U, double *V);
+
+/* svd_gpu with the same arguments and results, returning a status code
+ * instead of aborting.  No reference counterpart (the reference has only the
+ * aborting form, svd_gpu.c:53). */
+int svd_gpu_status(int m, int n, double *A, double *sigma, double *U,
+                   double *V);
+
+/*
+ * Device-pointer form of svd_gpu for callers whose data already lives in HBM
+ * (no reference counterpart: the reference copies host buffers in and out on
+ * every call, bidiag_par.c:298-301,405-418).
+ *   dA     m x n device matrix, leading dimension lda >= m; NOT modified
+ *   dsigma min(m,n) device values, ascending
+ *   dU     m x min(m,n) device matrix, leading dimension ldu >= m
+ *   dV     n x min(m,n) device matrix, leading dimension ldv >= n
+ *   stream a hipStream_t (NULL = the library's own stream); the call is
+ *          ordered on it and returns after the results are complete.
+ */
+int svd_gpu_dev(int m, int n, const double *dA, int lda, double *dsigma,
+                double *dU, int ldu, double *dV, int ldv, void *stream);
+
+/*
+ * Stand-alone bidiagonalization, the replacement of
+ *   void bidiag_par(int m, int n, double* A, double* alpha, double* beta)
+ * (bidiag_par.h:30, bidiag_par.c:34-450).  Host buffers.  A is overwritten
+ * with the unit-norm reflectors exactly as the reference stores them
+ * (columns below/on the diagonal: left, rows right of the diagonal: right);
+ * alpha gets min(m,n) entries, beta min(m,n)-1 entries.  Requires m >= n.
+ */
+int bidiag_gpu(int m, int n, double *A, double *alpha, double *beta);
+
+/*
+ * Singular values (ascending) of the n x n upper bidiagonal with diagonal
+ * alpha[n] and super-diagonal beta[n-1], by the GPU divide and conquer
+ * (replacement of GetSingularValues_Parallel, Calculations-Parallel.h:41,
+ * Calculations-Parallel.c:852).  refine != 0 also runs the per-value
+ * twisted-factorization refinement.  Host buffers.
+ */
+int bdsvd_values_gpu(int n, const double *alpha, const double *beta,
+                     double *sigma, int refine);
+
+/*
+ * Singular vectors of the n x n upper bidiagonal for its ascending singular
+ * values (replacement of CalcRightSingularVectors and
+ * RighttoLeftSingularVectors, parallel-twisted.h:18-19).  X and Y are n x n
+ * host matrices whose column i is the right / left vector of sigma[i];
+ * sigma is refined in place.  Host buffers.
+ */
+int bdsvd_vectors_gpu(int n, const double *alpha, const double *beta,
+                      double *sigma, double *X, double *Y);
+
+/* Milliseconds of the stages of the last successful call on this thread's
+ * process: [0] copy-in, [1] bidiagonalization, [2] singular values,
+ * [3] vectors, [4] back-transformation, [5] copy-out.  Returns the number of
+ * entries written (at most count). */
+int svd_gpu_last_timings(double *ms, int count);
+
+/* Human-readable description of the last failure (empty if none). */
+const char *svd_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -192,3 +192,19 @@ def bidiag_values(alpha, beta, square=True):
         B[np.arange(n), np.arange(n)] = alpha
         B[np.arange(n), np.arange(1, n + 1)] = beta[:n]
     return np.sort(np.linalg.svd(B, compute_uv=False))
+
+
+def lapack_dlasq1(d, e):
+    """Singular values (ascending) of the upper bidiagonal (d, e) by LAPACK
+    dlasq1 (dqds), accurate to high relative accuracy."""
+    fn = _lapack_fn("dlasq1", [_ip, _dp, _dp, _dp, _ip])
+    d = np.array(d, dtype=np.float64, copy=True)
+    n = len(d)
+    e = np.array(np.append(np.asarray(e, dtype=np.float64)[: n - 1], 0.0), copy=True)
+    work = np.zeros(4 * n + 4)
+    info = ctypes.c_int(0)
+    nn = ctypes.c_int(n)
+    fn(ctypes.byref(nn), _ptr(d), _ptr(e), _ptr(work), ctypes.byref(info))
+    if info.value != 0:
+        raise RuntimeError(f"dlasq1 info={info.value}")
+    return np.sort(d)

@@ -79,10 +79,26 @@ static void reflect_right(int ld, int l, int k, const double *w, double *C) {
   }
 }
 
+/* The first `steps` column/row steps of the main loop only (a bounded
+ * sample of the serial bidiagonalization for timing); steps < 0 = all. */
+static void bidiag_seq_steps(int m, int n, double *A, double *alpha,
+                             double *beta, int steps);
+
 void orc_bidiag_seq(int m, int n, double *A, double *alpha, double *beta) {
+  bidiag_seq_steps(m, n, A, alpha, beta, -1);
+}
+
+void orc_bidiag_sample(int m, int n, double *A, double *alpha, double *beta,
+                       int steps) {
+  bidiag_seq_steps(m, n, A, alpha, beta, steps);
+}
+
+static void bidiag_seq_steps(int m, int n, double *A, double *alpha,
+                             double *beta, int steps) {
   int mn = (m < n) ? m : n;
   long ld = m;
-  for (int i = 0; i < mn - 1; ++i) {
+  int last = (steps >= 0 && steps < mn - 1) ? steps : mn - 1;
+  for (int i = 0; i < last; ++i) {
     double *aii = A + i + i * ld;
     alpha[i] = make_reflector(m - i, aii, 1);
     reflect_left((int)ld, m - i, n - i - 1, aii, aii + ld);
@@ -97,6 +113,7 @@ void orc_bidiag_seq(int m, int n, double *A, double *alpha, double *beta) {
       row[0] = 0;
     }
   }
+  if (last < mn - 1) return;
   if (n >= m + 1) {
     /* wide: final row reflector, diagonal taken as is (bidiag.c:133-160) */
     double *row = A + (mn - 1) + mn * ld;

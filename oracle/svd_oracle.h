@@ -24,6 +24,11 @@ extern "C" {
  * min(m,n) entries, beta min(m,n)-1 (m >= n) or m (m < n). */
 void orc_bidiag_seq(int m, int n, double *A, double *alpha, double *beta);
 
+/* The first `steps` steps of orc_bidiag_seq's main loop (bidiag.c:73-131)
+ * and nothing else: a bounded sample of the serial reduction for timing. */
+void orc_bidiag_sample(int m, int n, double *A, double *alpha, double *beta,
+                       int steps);
+
 /* Singular values of the N x (N+1) upper bidiagonal [b1; b2] by the
  * reference's double divide and conquer.  Calculations-Parallel.c:852-874
  * and the routines it calls (:48-348, :351-588, :590-703, :706-850).
