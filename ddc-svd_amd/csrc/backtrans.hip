@@ -65,13 +65,17 @@ int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd, hi
 // T factor from the Gram matrix G = W^T W (nb x nb, ld nb), one group.
 __global__ __launch_bounds__(64) void bt_tfactor(int nb, const double* G, double* T) {
   __shared__ double Ts[NBB * NBB];
+  __shared__ double Gs[NBB * NBB];
   int r = threadIdx.x;
-  for (int e = r; e < nb * nb; e += 64) Ts[e] = 0.0;
+  for (int e = r; e < nb * nb; e += 64) {
+    Ts[e] = 0.0;
+    Gs[e] = G[e];
+  }
   __syncthreads();
   for (int c = 0; c < nb; ++c) {
     double acc = 0.0;
     if (r < c)
-      for (int q = r; q < c; ++q) acc += Ts[r + q * nb] * G[q + c * nb];
+      for (int q = r; q < c; ++q) acc += Ts[r + q * nb] * Gs[q + c * nb];
     __syncthreads();
     if (r < c) Ts[r + c * nb] = -2.0 * acc;
     if (r == c) Ts[c + c * nb] = 2.0;
@@ -80,16 +84,20 @@ __global__ __launch_bounds__(64) void bt_tfactor(int nb, const double* G, double
   for (int e = r; e < nb * nb; e += 64) T[e] = Ts[e];
 }
 
+constexpr int SPLITS = 16;
+
 void backtrans_plan(ArenaPlan& p, int m, int n) {
   p.add<double>((size_t)NBB * NBB * 2);        // G, T
   p.add<double>((size_t)NBB * n * 2);          // Z, Z2
+  p.add<double>((size_t)SPLITS * NBB * n);     // split-K partials
 }
 
 // Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C.
 // left: w_k = A[k:m, k];  right: w_k = A[k, k+1:n] acting on rows k+1..n-1.
 static int apply_blocks(bool left, int m, int n, const double* A, int lda,
                         int nref, double* C, int ldc, int ncol, double* G,
-                        double* T, double* Z, double* Z2, hipStream_t st) {
+                        double* T, double* Z, double* Z2, double* part,
+                        size_t part_cap, hipStream_t st) {
   const int nblk = (int)ceil_div(nref, NBB);
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int k0 = bi * NBB;
@@ -113,12 +121,12 @@ static int apply_blocks(bool left, int m, int n, const double* A, int lda,
     Wt = W;
     Wt.op = (W.op == 'N') ? 'T' : 'N';
     // G = W^T W
-    DDC_TRY(gemm_f64(nb, nb, rows, 1.0, Wt, W, 0.0, G, nb, st));
+    DDC_TRY(gemm_f64_split(nb, nb, rows, 1.0, Wt, W, 0.0, G, nb, part, part_cap, st));
     hipLaunchKernelGGL(bt_tfactor, dim3(1), dim3(64), 0, st, nb, G, T);
     DDC_LAUNCHED();
     // Z = W^T C ; Z2 = T Z ; C -= W Z2
     GemmOperand Cop{Csub, ldc, 'N', MASK_NONE, 0};
-    DDC_TRY(gemm_f64(nb, ncol, rows, 1.0, Wt, Cop, 0.0, Z, nb, st));
+    DDC_TRY(gemm_f64_split(nb, ncol, rows, 1.0, Wt, Cop, 0.0, Z, nb, part, part_cap, st));
     GemmOperand Top{T, nb, 'N', MASK_NONE, 0};
     GemmOperand Zop{Z, nb, 'N', MASK_NONE, 0};
     DDC_TRY(gemm_f64(nb, ncol, nb, 1.0, Top, Zop, 0.0, Z2, nb, st));
@@ -136,7 +144,9 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   double* T = G + NBB * NBB;
   double* Z = ws.take<double>((size_t)NBB * n * 2);
   double* Z2 = Z + (size_t)NBB * n;
-  if (!Z) return DDC_E_NOMEM;
+  const size_t part_cap = (size_t)SPLITS * NBB * n;
+  double* part = ws.take<double>(part_cap);
+  if (!part) return DDC_E_NOMEM;
   dim3 tb(32, 8);
   dim3 gu((unsigned)ceil_div(n, 32), (unsigned)ceil_div(m, 32));
   hipLaunchKernelGGL(bt_unpack, gu, tb, 0, st, n, m, Ut, ldb, U, ldu);
@@ -145,9 +155,10 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   hipLaunchKernelGGL(bt_unpack, gv, tb, 0, st, n, n, Vt, ldb, V, ldv);
   DDC_LAUNCHED();
   // left: reflectors 0..n-1 (the last one is the 1 x 1 sign flip when m == n)
-  DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, T, Z, Z2, st));
+  DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, T, Z, Z2, part, part_cap, st));
   // right: reflectors 0..n-3 (row n-2 holds no reflection)
-  if (n > 2) DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, T, Z, Z2, st));
+  if (n > 2)
+    DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, T, Z, Z2, part, part_cap, st));
   return DDC_OK;
 }
 

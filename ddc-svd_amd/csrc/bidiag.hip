@@ -23,17 +23,19 @@
 // over the trailing matrix: 16 bytes per trailing element per step.  After
 // NB steps the trailing matrix gets A0 -= U Y^T + X V^T on the FP64 matrix
 // cores (gemm_f64_2), one read and one write.
+#include <algorithm>
+
 #include "ddc_common.h"
 
 namespace ddc {
 
 constexpr int NB = 32;        // panel width
-constexpr int RC = 512;       // rows per left_gemv chunk (2 per thread)
+constexpr int RC = 2048;      // rows per left_gemv chunk (8 per thread)
 constexpr int CBL = 16;       // columns per left_gemv work-group
 constexpr int RBR = 512;      // rows per right_gemv block (2 per thread)
 constexpr int CCR = 128;      // columns per right_gemv chunk
 constexpr int DPW = 8;        // panel dot products per dot work-group
-constexpr int FIN = 256;      // items per finalize work-group
+constexpr int FIN = 64;       // items per finalize work-group (one wave)
 constexpr int DSTRIDE = 2 * NB + 1;
 
 struct StepArgs {
@@ -70,6 +72,22 @@ __device__ __forceinline__ double sum_partials(const double* part, int cnt) {
   double s = 0.0;
   for (int i = (int)(threadIdx.x & 63); i < cnt; i += 64) s += part[i];
   return wave_sum(s);
+}
+
+// sum_{q<cnt} p[q*stride] in a fixed order, with the loads issued in
+// batches of 8 so that their latencies overlap.
+__device__ __forceinline__ double sum_strided(const double* p, int cnt, int64_t stride) {
+  double acc = 0.0;
+  int q = 0;
+  for (; q + 8 <= cnt; q += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(q + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; q < cnt; ++q) acc += p[(int64_t)q * stride];
+  return acc;
 }
 
 // Householder data of the vector x with |x|^2 = sumsq and first entry x1,
@@ -121,41 +139,63 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
   Refl R;
   R.make(s_scal, a.cvec[k]);
 
+  // rows of this thread: r0 + 512 q + {0, 1}, q = 0..3 (coalesced double2)
   const int r0 = rb0 + blockIdx.y * RC + 2 * tid;
-  const bool inr = r0 < a.lda;
-  double c0 = 0.0, c1 = 0.0;
-  if (inr) {
-    if (r0 >= k && r0 < a.m) c0 = a.cvec[r0];
-    if (r0 + 1 >= k && r0 + 1 < a.m) c1 = a.cvec[r0 + 1];
+  double cv[8];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int r = r0 + 512 * q + h;
+      cv[2 * q + h] = (r >= k && r < a.m) ? a.cvec[r] : 0.0;
+    }
   }
 
   if ((int)blockIdx.x < ncb) {
-    const double u0 = (r0 >= k && r0 < a.m) ? R.vec(c0, r0 == k) : 0.0;
-    const double u1 = (r0 + 1 >= k && r0 + 1 < a.m) ? R.vec(c1, r0 + 1 == k) : 0.0;
+    double u[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = r0 + 512 * q + h;
+        u[2 * q + h] = (r >= k && r < a.m) ? R.vec(cv[2 * q + h], r == k) : 0.0;
+      }
     const int j0 = k + 1 + blockIdx.x * CBL;
     const int jn = min(CBL, a.n - j0);
     double acc[CBL];
 #pragma unroll
-    for (int q = 0; q < CBL; ++q) {
-      acc[q] = 0.0;
-      if (q < jn && inr) {
-        double2 v = *reinterpret_cast<const double2*>(a.A + r0 + (int64_t)(j0 + q) * a.lda);
-        acc[q] = v.x * u0 + v.y * u1;
+    for (int c = 0; c < CBL; ++c) {
+      acc[c] = 0.0;
+      if (c < jn) {
+        const double* col = a.A + (int64_t)(j0 + c) * a.lda;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = r0 + 512 * q;
+          if (r < a.lda) {
+            double2 v = *reinterpret_cast<const double2*>(col + r);
+            acc[c] += v.x * u[2 * q] + v.y * u[2 * q + 1];
+          }
+        }
       }
     }
 #pragma unroll
-    for (int q = 0; q < CBL; ++q) {
-      double s = wave_sum(acc[q]);
-      if (lane == 0) red[wave][q] = s;
+    for (int c = 0; c < CBL; ++c) {
+      double s = wave_sum(acc[c]);
+      if (lane == 0) red[wave][c] = s;
     }
     __syncthreads();
     if (tid < jn)
       a.Lpart[(int64_t)blockIdx.y * a.ldy + j0 + tid] =
           red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
     // the reflector itself goes to its final place, column k
-    if (blockIdx.x == 0 && inr) {
-      if (r0 >= k && r0 < a.m) a.A[r0 + (int64_t)k * a.lda] = u0;
-      if (r0 + 1 >= k && r0 + 1 < a.m) a.A[r0 + 1 + (int64_t)k * a.lda] = u1;
+    if (blockIdx.x == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int r = r0 + 512 * q + h;
+          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = u[2 * q + h];
+        }
       if (blockIdx.y == 0 && tid == 0) a.alpha[k] = R.diag();
     }
   } else {
@@ -164,20 +204,26 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
     const int nd = 2 * a.t;
     double acc[DPW];
 #pragma unroll
-    for (int q = 0; q < DPW; ++q) {
-      acc[q] = 0.0;
-      int s = q0 + q;
-      if (s < nd && inr) {
+    for (int e = 0; e < DPW; ++e) {
+      acc[e] = 0.0;
+      const int s = q0 + e;
+      if (s < nd) {
         const double* col = (s < a.t) ? a.A + (int64_t)(a.p + s) * a.lda
                                       : a.X + (int64_t)(s - a.t) * a.lda;
-        double2 v = *reinterpret_cast<const double2*>(col + r0);
-        acc[q] = v.x * c0 + v.y * c1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = r0 + 512 * q;
+          if (r < a.lda) {
+            double2 v = *reinterpret_cast<const double2*>(col + r);
+            acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
+          }
+        }
       }
     }
 #pragma unroll
-    for (int q = 0; q < DPW; ++q) {
-      double s = wave_sum(acc[q]);
-      if (lane == 0) red[wave][q] = s;
+    for (int e = 0; e < DPW; ++e) {
+      double s = wave_sum(acc[e]);
+      if (lane == 0) red[wave][e] = s;
     }
     __syncthreads();
     if (tid < DPW && q0 + tid < nd)
@@ -186,25 +232,17 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
   }
 }
 
-// left_fin: one thread per trailing column j in [k+1, n).
-__global__ __launch_bounds__(256) void bd_left_fin(StepArgs a, int nchL) {
-  __shared__ double red[4];
-  __shared__ double s_scal;
+// left_fin: one thread per trailing column j in [k+1, n), one wave per group.
+__global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs a, int nchL) {
   __shared__ double dd[2 * NB];  // u_s^T u, x_s^T u
   __shared__ double uk[NB + 1];  // u_s[k], s <= t
   __shared__ double xk[NB];      // x_s[k], s < t
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int k = a.k, t = a.t;
-  if (wave == 0) {
-    double s = sum_partials(a.cnorm, a.ncn);
-    if (lane == 0) s_scal = s;
-  }
-  __syncthreads();
   Refl R;
-  R.make(s_scal, a.cvec[k]);
+  R.make(sum_partials(a.cnorm, a.ncn), a.cvec[k]);
   if (tid < 2 * t) {
-    double s = 0.0;
-    for (int ch = 0; ch < nchL; ++ch) s += a.Ldot[(int64_t)ch * (2 * NB) + tid];
+    double s = sum_strided(a.Ldot + tid, nchL, 2 * NB);
     double w1 = (tid < t) ? a.A[k + (int64_t)(a.p + tid) * a.lda]
                           : a.X[k + (int64_t)(tid - t) * a.lda];
     dd[tid] = R.dot(s, w1);
@@ -219,21 +257,34 @@ __global__ __launch_bounds__(256) void bd_left_fin(StepArgs a, int nchL) {
   const int j = k + 1 + blockIdx.x * FIN + tid;
   double rr = 0.0;
   if (j < a.n) {
-    double g = 0.0;
-    for (int ch = 0; ch < nchL; ++ch) g += a.Lpart[(int64_t)ch * a.ldy + j];
-    double y = g;
-    for (int s = 0; s < t; ++s)
-      y -= a.Y[j + (int64_t)s * a.ldy] * dd[s] + a.V[j + (int64_t)s * a.ldy] * dd[t + s];
+    double y = sum_strided(a.Lpart + j, nchL, a.ldy);
+    double r = a.A[k + (int64_t)j * a.lda];
+    const double* Yj = a.Y + j;
+    const double* Vj = a.V + j;
+    for (int s0 = 0; s0 < t; s0 += 8) {
+      double ys[8], vs[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = min(s0 + u, t - 1);  // clamped: in bounds, unused past t
+        ys[u] = Yj[(int64_t)s * a.ldy];
+        vs[u] = Vj[(int64_t)s * a.ldy];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = s0 + u;
+        if (s < t) {
+          y -= ys[u] * dd[s] + vs[u] * dd[t + s];
+          r -= uk[s] * ys[u] + xk[s] * vs[u];
+        }
+      }
+    }
     y *= 2.0;
     a.Y[j + (int64_t)t * a.ldy] = y;
-    double r = a.A[k + (int64_t)j * a.lda];
-    for (int s = 0; s < t; ++s)
-      r -= uk[s] * a.Y[j + (int64_t)s * a.ldy] + xk[s] * a.V[j + (int64_t)s * a.ldy];
     r -= uk[t] * y;
     a.rvec[j] = r;
     rr = r * r;
   }
-  double s = block_sum256(rr, red);
+  double s = wave_sum(rr);
   if (tid == 0) a.rnorm[blockIdx.x] = s;
 }
 
@@ -319,28 +370,21 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb
   }
 }
 
-// right_fin: one thread per row in [k+1, m).  noright: the row of M has a
-// single entry, taken as beta with no reflection (bidiag.c:125-129).
-__global__ __launch_bounds__(256) void bd_right_fin(StepArgs a, int nccR, int noright) {
-  __shared__ double red[4];
-  __shared__ double s_scal;
+// right_fin: one thread per row in [k+1, m), one wave per group.  noright:
+// the row of M has a single entry, taken as beta with no reflection
+// (bidiag.c:125-129).
+__global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs a, int nccR, int noright) {
   __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
   __shared__ double yk1[NB];       // Y[k+1, s], s <= t
   __shared__ double vk1[NB];       // V[k+1, s], s <= t
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int k = a.k, t = a.t;
   Refl R;
   R.nrm = R.sgn = R.rl = 0.0;
   if (!noright) {
-    if (wave == 0) {
-      double s = sum_partials(a.rnorm, a.nrn);
-      if (lane == 0) s_scal = s;
-    }
-    __syncthreads();
-    R.make(s_scal, a.rvec[k + 1]);
+    R.make(sum_partials(a.rnorm, a.nrn), a.rvec[k + 1]);
     if (tid < 2 * t + 1) {
-      double s = 0.0;
-      for (int cc = 0; cc < nccR; ++cc) s += a.Rdot[(int64_t)cc * DSTRIDE + tid];
+      double s = sum_strided(a.Rdot + tid, nccR, DSTRIDE);
       double w1 = (tid <= t) ? a.Y[(k + 1) + (int64_t)tid * a.ldy]
                              : a.V[(k + 1) + (int64_t)(tid - t - 1) * a.ldy];
       dyv[tid] = R.dot(s, w1);
@@ -352,29 +396,44 @@ __global__ __launch_bounds__(256) void bd_right_fin(StepArgs a, int nccR, int no
   __syncthreads();
 
   const int row = k + 1 + blockIdx.x * FIN + tid;
+  const bool next = k + 1 < a.n;
   double cc2 = 0.0;
   if (row < a.m) {
     double x = 0.0;
+    double c = next ? a.A[row + (int64_t)(k + 1) * a.lda] : 0.0;
+    if (!noright) x = sum_strided(a.Rpart + row, nccR, a.lda);
+    const double* Ur = a.A + row + (int64_t)a.p * a.lda;
+    const double* Xr = a.X + row;
+    for (int s0 = 0; s0 < t; s0 += 8) {
+      double us[8], xs[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = min(s0 + u, t - 1);  // clamped: in bounds, unused past t
+        us[u] = Ur[(int64_t)s * a.lda];
+        xs[u] = Xr[(int64_t)s * a.lda];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = s0 + u;
+        if (s < t) {
+          if (!noright) x -= us[u] * dyv[s] + xs[u] * dyv[t + 1 + s];
+          c -= us[u] * yk1[s] + xs[u] * vk1[s];
+        }
+      }
+    }
+    double ut = Ur[(int64_t)t * a.lda];
     if (!noright) {
-      double h = 0.0;
-      for (int cc = 0; cc < nccR; ++cc) h += a.Rpart[(int64_t)cc * a.lda + row];
-      x = h;
-      for (int s = 0; s <= t; ++s) x -= a.A[row + (int64_t)(a.p + s) * a.lda] * dyv[s];
-      for (int s = 0; s < t; ++s) x -= a.X[row + (int64_t)s * a.lda] * dyv[t + 1 + s];
+      x -= ut * dyv[t];
       x *= 2.0;
     }
     a.X[row + (int64_t)t * a.lda] = x;
-    if (k + 1 < a.n) {
-      double c = a.A[row + (int64_t)(k + 1) * a.lda];
-      for (int s = 0; s < t; ++s)
-        c -= a.A[row + (int64_t)(a.p + s) * a.lda] * yk1[s] +
-             a.X[row + (int64_t)s * a.lda] * vk1[s];
-      c -= a.A[row + (int64_t)(a.p + t) * a.lda] * yk1[t] + x * vk1[t];
+    if (next) {
+      c -= ut * yk1[t] + x * vk1[t];
       a.cvec[row] = c;
       cc2 = c * c;
     }
   }
-  double s = block_sum256(cc2, red);
+  double s = wave_sum(cc2);
   if (tid == 0) a.cnorm[blockIdx.x] = s;
   if (noright && blockIdx.x == 0 && tid == 0) {
     a.beta[k] = a.rvec[k + 1];
@@ -412,7 +471,7 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
   d.ldy = (int)round_up(n, 16);
   d.nchL_max = (int)ceil_div(lda, RC) + 1;
   d.nccR_max = (int)ceil_div(n, CCR) + 1;
-  d.ncn_max = (int)ceil_div(m, 256) + 1;
+  d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(m, FIN)) + 1;
   d.nrn_max = (int)ceil_div(n, FIN) + 1;
   return d;
 }
@@ -482,13 +541,13 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
     hipLaunchKernelGGL(bd_left_gemv, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
     DDC_LAUNCHED();
     const int nfl = (int)ceil_div(n - k - 1, FIN);
-    hipLaunchKernelGGL(bd_left_fin, dim3(nfl), dim3(256), 0, st, a, nchL);
+    hipLaunchKernelGGL(bd_left_fin, dim3(nfl), dim3(FIN), 0, st, a, nchL);
     DDC_LAUNCHED();
     a.nrn = nfl;
     // right reflector of row k and the right GEMV
     const int nfr = (int)ceil_div(m - k - 1, FIN);
     if (k == n - 2) {
-      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(256), 0, st, a, 0, 1);
+      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
       DDC_LAUNCHED();
     } else {
       const int rbr = ((k + 1) / 64) * 64;
@@ -497,7 +556,7 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
       const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
       hipLaunchKernelGGL(bd_right_gemv, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
       DDC_LAUNCHED();
-      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(256), 0, st, a, ncc, 0);
+      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();
     }
     a.ncn = nfr;

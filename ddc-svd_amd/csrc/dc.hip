@@ -23,9 +23,11 @@
 // GPU mapping: the recursion tree is planned on the host and processed
 // bottom-up, one level per round.  Per level five kernels run over all
 // merges of the level: prep (one work-group per merge: rank-merge of the two
-// sorted child spectra, deflation, compaction), roots (one lane per root),
-// zhat (one lane per pole), rows (one lane per root: first/last rows of the
-// new singular vectors) and assemble (one work-group per merge).
+// sorted child spectra, deflation, compaction), roots (one wave per root),
+// zhat (one wave per pole), rows (one wave per root: first/last rows of the
+// new singular vectors) and assemble (one work-group per merge).  The roots
+// kernel gives a whole wave to each root: its lanes split the O(K) secular
+// sum, so the top merge of an n = 8192 problem fills the GPU.
 #include <algorithm>
 #include <vector>
 
@@ -315,42 +317,59 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
   }
 }
 
-// secular function 1 + sum z_j^2 / ((d_j - w)(d_j + w)) at w = d_o + tau
-__device__ __forceinline__ double secular(const double* d, const double* z,
-                                          int K, int o, double tau) {
+// Wave-cooperative secular function 1 + sum z_j^2 / ((d_j - w)(d_j + w)) at
+// w = d_o + tau: lane l takes j = l, l+64, ...; fixed-order wave sum, so every
+// lane gets the same value.
+__device__ __forceinline__ double secular_wave(const double* d, const double* z,
+                                               int K, int o, double tau, int lane) {
   const double dO = d[o];
-  double f = 1.0;
-  for (int j = 0; j < K; ++j) {
+  double f = 0.0;
+  for (int j = lane; j < K; j += 64) {
     double zj = z[j];
     double dj = d[j];
     f += (zj * zj) / (((dj - dO) - tau) * (dj + dO + tau));
   }
-  return f;
+  return 1.0 + wave_sum(f);
 }
 
+__device__ __forceinline__ double wave_prod(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v *= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Work items: each group of 256 threads takes 4 consecutive items of one
+// merge, one wave per item.
+#define DC_ITEM()                                         \
+  const int2 it = B.items[ibase + blockIdx.x];            \
+  const int mi = it.x;                                    \
+  const DcMerge M = B.merges[mi];                         \
+  const int lane = threadIdx.x & 63;                      \
+  const int item = it.y + (threadIdx.x >> 6);
+
 // ---------------------------------------------------------------------------
-// roots: one lane per secular root
+// roots: one wave per secular root
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void dc_roots(DcBufs B, int ibase) {
-  const int2 it = B.items[ibase + blockIdx.x];
-  const int mi = it.x;
-  const DcMerge M = B.merges[mi];
+  DC_ITEM();
   const DcMergeState st = B.ms[mi];
-  const int i = it.y + threadIdx.x;
+  const int i = item;
   const int K = st.kept;
   if (i >= K) return;
   const double* d = B.kd + M.off;
   const double* z = B.kz + M.off;
-  int o;
-  double A, sgn;
   if (K == 1) {
-    B.org[M.off] = 0;
-    B.tau[M.off] = fabs(z[0]);
+    if (lane == 0) {
+      B.org[M.off] = 0;
+      B.tau[M.off] = fabs(z[0]);
+    }
     return;
   }
+  int o;
+  double A, sgn;
   if (i < K - 1) {
     double mid = 0.5 * (d[i] + d[i + 1]);
-    double fm = secular(d, z, K, i, mid - d[i]);
+    double fm = secular_wave(d, z, K, i, mid - d[i], lane);
     if (fm >= 0.0) {
       o = i;
       A = mid - d[i];
@@ -373,51 +392,56 @@ __global__ __launch_bounds__(256) void dc_roots(DcBufs B, int ibase) {
     else if (hi > 4.0 * lo) t = sqrt(lo * hi);
     else t = 0.5 * (lo + hi);
     if (!(t > lo && t < hi)) break;
-    double f = secular(d, z, K, o, sgn * t);
+    double f = secular_wave(d, z, K, o, sgn * t, lane);
     // origin below the root: f < 0 -> root further out; origin above: f > 0
     bool further = (sgn > 0.0) ? (f < 0.0) : (f > 0.0);
     if (further) lo = t; else hi = t;
     if (hi - lo <= 2.0 * EPS * lo) break;
   }
-  B.org[M.off + i] = o;
-  B.tau[M.off + i] = sgn * 0.5 * (lo + hi);
+  if (lane == 0) {
+    B.org[M.off + i] = o;
+    B.tau[M.off + i] = sgn * 0.5 * (lo + hi);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// zhat: one lane per kept pole.  zhat_j^2 = (w_{K-1}^2 - d_j^2)
+// zhat: one wave per kept pole.  zhat_j^2 = (w_{K-1}^2 - d_j^2)
 //   * prod_{k<j} (w_k^2 - d_j^2)/(d_k^2 - d_j^2)
 //   * prod_{j<=k<K-1} (w_k^2 - d_j^2)/(d_{k+1}^2 - d_j^2)
 // with w_k - d_j = (d_{org(k)} - d_j) + tau_k.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void dc_zhat(DcBufs B, int ibase) {
-  const int2 it = B.items[ibase + blockIdx.x];
-  const int mi = it.x;
-  const DcMerge M = B.merges[mi];
+  DC_ITEM();
   const int K = B.ms[mi].kept;
-  const int j = it.y + threadIdx.x;
+  const int j = item;
   if (j >= K) return;
   const double* d = B.kd + M.off;
   const int* org = B.org + M.off;
   const double* tau = B.tau + M.off;
   const double dj = d[j];
-  auto wm = [&](int k) { return (d[org[k]] - dj) + tau[k]; };  // w_k - d_j
-  auto wp = [&](int k) { return d[org[k]] + tau[k] + dj; };    // w_k + d_j
-  double prod = wm(K - 1) * wp(K - 1);
-  for (int k = 0; k < j; ++k) prod *= (wm(k) * wp(k)) / ((d[k] - dj) * (d[k] + dj));
-  for (int k = j; k < K - 1; ++k)
-    prod *= (wm(k) * wp(k)) / ((d[k + 1] - dj) * (d[k + 1] + dj));
-  B.zh[M.off + j] = copysign(sqrt(fabs(prod)), B.kz[M.off + j]);
+  double prod = 1.0;
+  for (int k = lane; k < K - 1; k += 64) {
+    double wm = (d[org[k]] - dj) + tau[k];  // w_k - d_j
+    double wp = d[org[k]] + tau[k] + dj;    // w_k + d_j
+    double dk = (k < j) ? d[k] : d[k + 1];
+    prod *= (wm * wp) / ((dk - dj) * (dk + dj));
+  }
+  prod = wave_prod(prod);
+  if (lane == 0) {
+    double wm = (d[org[K - 1]] - dj) + tau[K - 1];
+    double wp = d[org[K - 1]] + tau[K - 1] + dj;
+    prod *= wm * wp;
+    B.zh[M.off + j] = copysign(sqrt(fabs(prod)), B.kz[M.off + j]);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// rows: one lane per root: first/last rows of the merged right vectors.
+// rows: one wave per root: first/last rows of the merged right vectors.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void dc_rows(DcBufs B, int ibase) {
-  const int2 it = B.items[ibase + blockIdx.x];
-  const int mi = it.x;
-  const DcMerge M = B.merges[mi];
+  DC_ITEM();
   const int K = B.ms[mi].kept;
-  const int i = it.y + threadIdx.x;
+  const int i = item;
   if (i >= K) return;
   const double* d = B.kd + M.off;
   const double* zh = B.zh + M.off;
@@ -428,15 +452,28 @@ __global__ __launch_bounds__(256) void dc_rows(DcBufs B, int ibase) {
   const double dO = d[o];
   const double w = dO + ti;
   double nn = 0.0, fs = 0.0, ls = 0.0;
-  for (int j = 0; j < K; ++j) {
-    double v = zh[j] / (((d[j] - dO) - ti) * (d[j] + w));
-    nn += v * v;
-    fs += F[j] * v;
-    ls += L[j] * v;
+  if (K > 1) {
+    for (int j = lane; j < K; j += 64) {
+      double v = zh[j] / (((d[j] - dO) - ti) * (d[j] + w));
+      nn += v * v;
+      fs += F[j] * v;
+      ls += L[j] * v;
+    }
+    nn = wave_sum(nn);
+    fs = wave_sum(fs);
+    ls = wave_sum(ls);
   }
-  double inv = 1.0 / sqrt(nn);
-  B.fn[M.off + i] = fs * inv;
-  B.ln[M.off + i] = ls * inv;
+  if (lane != 0) return;
+  if (nn > 0.0 && isfinite(nn)) {
+    double inv = 1.0 / sqrt(nn);
+    B.fn[M.off + i] = fs * inv;
+    B.ln[M.off + i] = ls * inv;
+  } else {
+    // a single kept pole (the arrow is 1 x 1) or a vector lost to
+    // underflow: the singular vector is the pole's own unit vector
+    B.fn[M.off + i] = F[o];
+    B.ln[M.off + i] = L[o];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -499,6 +536,14 @@ static int build_tree(std::vector<DcNode>& nodes, int off, int N) {
   return id;
 }
 
+// Upper bound on the work items: per level at most n/4 + (merges of the
+// level) groups, at most log2(n) + 2 levels.
+static size_t dc_item_cap(size_t nn) {
+  size_t lv = 2;
+  while (((size_t)1 << lv) < nn) ++lv;
+  return (nn / 4 + 2) * (lv + 2) + nn + 64;
+}
+
 void dc_plan(ArenaPlan& p, int n) {
   size_t nn = (size_t)n + 1;
   p.add<double>(nn * 3);          // sig fr lr
@@ -511,7 +556,7 @@ void dc_plan(ArenaPlan& p, int n) {
   p.add<double>(nn * 4);          // tau zh fn ln
   p.add<DcMergeState>(nn);
   p.add<DcMerge>(nn);
-  p.add<int2>(2 * nn + 64);
+  p.add<int2>(dc_item_cap(nn));
   p.add<int3>(nn + 1);
 }
 
@@ -543,7 +588,7 @@ int dc_run(int n, const double* a, const double* b, double* sigma, Arena& ws,
   for (int h = 1; h <= H; ++h) {
     istart[h] = (int)items.size();
     for (int mi = mstart[h]; mi < mstart[h + 1]; ++mi)
-      for (int i0 = 0; i0 < merges[mi].N; i0 += 256) items.push_back(int2{mi, i0});
+      for (int i0 = 0; i0 < merges[mi].N; i0 += 4) items.push_back(int2{mi, i0});
   }
   istart[H + 1] = (int)items.size();
 
@@ -575,10 +620,10 @@ int dc_run(int n, const double* a, const double* b, double* sigma, Arena& ws,
   B.ln = B.fn + nn;
   B.ms = ws.take<DcMergeState>(nn);
   DcMerge* dmerges = ws.take<DcMerge>(nn);
-  int2* ditems = ws.take<int2>(2 * nn + 64);
+  int2* ditems = ws.take<int2>(dc_item_cap(nn));
   int3* dleaves = ws.take<int3>(nn + 1);
   if (!dleaves) return DDC_E_NOMEM;
-  if (merges.size() > nn || items.size() > 2 * nn + 64 || leaves.size() > nn + 1)
+  if (merges.size() > nn || items.size() > dc_item_cap(nn) || leaves.size() > nn + 1)
     return DDC_E_ARG;
   B.merges = dmerges;
   B.items = ditems;

@@ -123,6 +123,11 @@ struct GemmOperand {
 };
 int gemm_f64(int M, int N, int K, double alpha, GemmOperand A, GemmOperand B,
              double beta, double* C, int ldc, hipStream_t st);
+// The same, split over K when the output has few tiles: `part` (capacity
+// part_cap doubles) holds the partial products, summed in a fixed order.
+int gemm_f64_split(int M, int N, int K, double alpha, GemmOperand A, GemmOperand B,
+                   double beta, double* C, int ldc, double* part, size_t part_cap,
+                   hipStream_t st);
 // Two operand pairs accumulated into one pass over C.
 int gemm_f64_2(int M, int N, double alpha, GemmOperand A1, GemmOperand B1,
                int K1, GemmOperand A2, GemmOperand B2, int K2, double beta,

@@ -44,6 +44,13 @@ def load():
     """Load the built library (raises if it was not built)."""
     global _lib
     if _lib is None:
+        try:
+            # PyTorch-ROCm carries its own HIP runtime (same soname); loading it
+            # first makes this library bind to that one instance instead of
+            # bringing a second runtime into the process.
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise FileNotFoundError(
                 f"{LIB_PATH} is missing: build it with `python ddc-svd_amd/build.py`")
