@@ -5,10 +5,11 @@
 // time (BLAS1).  Here the reflectors stored in A are grouped into blocks of
 // NBB and applied in compact WY form,
 //     H_k0 ... H_{k0+NBB-1} = I - W T W^T,   T upper triangular,
-// built for the unit-norm convention H = I - 2 w w^T (tau = 2):
-//     T(c, c) = 2,  T(0:c, c) = -2 T(0:c, 0:c) (W(:, 0:c)^T w_c).
-// Each block costs three GEMMs on the whole set of vectors at once:
-//     Z = W^T C,  Z = T Z,  C -= W Z.
+// for the unit-norm convention H = I - 2 w w^T (tau = 2).  T is never formed:
+// its inverse is T^{-1} = I/2 + strict_upper(W^T W) (the identity behind
+// the recursive T-factor of LAPACK dlarft), so a block costs
+//     Z = W^T C  (MFMA GEMM),  Z = T Z by one upper-triangular solve per
+//     column of Z,  C -= W Z  (MFMA GEMM).
 // Left reflectors are the columns A[k:m, k]; right reflectors are the rows
 // A[k, k+1:n] (leading dimension n for V -- the reference indexes the
 // transposed copy with m, bidiag_par.c:1034-1036, wrong when m != n).
@@ -62,26 +63,31 @@ int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd, hi
   return DDC_OK;
 }
 
-// T factor from the Gram matrix G = W^T W (nb x nb, ld nb), one group.
-__global__ __launch_bounds__(64) void bt_tfactor(int nb, const double* G, double* T) {
-  __shared__ double Ts[NBB * NBB];
+// Z <- T Z with T^{-1} = I/2 + strict_upper(G), G = W^T W (nb x nb, ld nb):
+// back substitution z_i = 2 (z_i - sum_{j>i} G(i,j) z_j), one thread per
+// column of Z (ld nb).
+__global__ __launch_bounds__(64) void bt_trsm(int nb, int ncol, const double* G, double* Z) {
   __shared__ double Gs[NBB * NBB];
-  int r = threadIdx.x;
-  for (int e = r; e < nb * nb; e += 64) {
-    Ts[e] = 0.0;
-    Gs[e] = G[e];
-  }
+  for (int e = threadIdx.x; e < nb * nb; e += 64) Gs[e] = G[e];
   __syncthreads();
-  for (int c = 0; c < nb; ++c) {
-    double acc = 0.0;
-    if (r < c)
-      for (int q = r; q < c; ++q) acc += Ts[r + q * nb] * Gs[q + c * nb];
-    __syncthreads();
-    if (r < c) Ts[r + c * nb] = -2.0 * acc;
-    if (r == c) Ts[c + c * nb] = 2.0;
-    __syncthreads();
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncol) return;
+  double z[NBB];
+  double* zc = Z + (int64_t)c * nb;
+#pragma unroll
+  for (int i = 0; i < NBB; ++i) z[i] = (i < nb) ? zc[i] : 0.0;
+#pragma unroll
+  for (int i = NBB - 1; i >= 0; --i) {
+    if (i >= nb) continue;
+    double acc = z[i];
+#pragma unroll
+    for (int j = i + 1; j < NBB; ++j)
+      if (j < nb) acc -= Gs[i + j * nb] * z[j];
+    z[i] = 2.0 * acc;
   }
-  for (int e = r; e < nb * nb; e += 64) T[e] = Ts[e];
+#pragma unroll
+  for (int i = 0; i < NBB; ++i)
+    if (i < nb) zc[i] = z[i];
 }
 
 constexpr int SPLITS = 16;
@@ -120,17 +126,14 @@ static int apply_blocks(bool left, int m, int n, const double* A, int lda,
     if (rows <= 0) continue;
     Wt = W;
     Wt.op = (W.op == 'N') ? 'T' : 'N';
-    // G = W^T W
+    // G = W^T W ; Z = W^T C ; Z = T Z ; C -= W Z
     DDC_TRY(gemm_f64_split(nb, nb, rows, 1.0, Wt, W, 0.0, G, nb, part, part_cap, st));
-    hipLaunchKernelGGL(bt_tfactor, dim3(1), dim3(64), 0, st, nb, G, T);
-    DDC_LAUNCHED();
-    // Z = W^T C ; Z2 = T Z ; C -= W Z2
     GemmOperand Cop{Csub, ldc, 'N', MASK_NONE, 0};
     DDC_TRY(gemm_f64_split(nb, ncol, rows, 1.0, Wt, Cop, 0.0, Z, nb, part, part_cap, st));
-    GemmOperand Top{T, nb, 'N', MASK_NONE, 0};
-    GemmOperand Zop{Z, nb, 'N', MASK_NONE, 0};
-    DDC_TRY(gemm_f64(nb, ncol, nb, 1.0, Top, Zop, 0.0, Z2, nb, st));
-    GemmOperand Z2op{Z2, nb, 'N', MASK_NONE, 0};
+    hipLaunchKernelGGL(bt_trsm, dim3((unsigned)ceil_div(ncol, 64)), dim3(64), 0, st, nb, ncol,
+                       G, Z);
+    DDC_LAUNCHED();
+    GemmOperand Z2op{Z, nb, 'N', MASK_NONE, 0};
     DDC_TRY(gemm_f64(rows, ncol, nb, -1.0, W, Z2op, 1.0, Csub, ldc, st));
   }
   return DDC_OK;

@@ -36,14 +36,13 @@ struct TwArgs {
   double* Vt;            // v_i[j] at j*ldv + i
   double* Ut;            // u_i[j] at j*ldv + i
   int ldv;
-  double* Lp;            // scratch [n][T]: stationary multipliers L+
-  double* S;             // scratch [n][T]: stationary s_i
-  double* Um;            // scratch [n][T]: progressive multipliers U-
-  int T;                 // scratch stride (2n)
+  double* Sck;           // checkpoints of s every TW_CK steps [nck][T]
+  double* Pck;           // checkpoints of p every TW_CK steps [nck][T]
+  int T;                 // checkpoint stride (2n)
   double smax;
 };
 
-constexpr int TW_CH = 256;  // qd entries staged in LDS per chunk
+constexpr int TW_CK = 64;  // checkpoint interval = LDS chunk
 
 // qd data for B and its reversal
 __global__ void tw_qd(int n, const double* a, const double* b, double* qd) {
@@ -74,29 +73,49 @@ __device__ __forceinline__ double guard(double x, double pivmin) {
 }
 
 // One wave per group, all lanes on the same representation; lane = one
-// singular value.  All lanes walk the index range in lockstep, so the qd
-// arrays (shared by the lanes) are staged through LDS chunk by chunk and the
-// per-lane scratch accesses (column-interleaved, stride T) are coalesced.
+// singular value.  The lanes walk the index range in lockstep, chunk by
+// chunk: the chunk's qd entries are staged in LDS, and the recurrences the
+// twisted factorization needs in the opposite direction are recomputed for
+// the chunk from checkpoints (every TW_CK steps) into a per-lane LDS column.
+// Nothing of size n per lane is stored except the vector itself.
+//
+// Recurrences (L D L^T - lam I, d = a^2, ld = a b, lld = b^2):
+//   stationary  s_0 = -lam, D+_i = d_i + s_i, s_{i+1} = lld_i s_i / D+_i - lam
+//   progressive p_{n-1} = d_{n-1} - lam, D-_{i+1} = lld_i + p_{i+1},
+//               p_i = d_i p_{i+1} / D-_{i+1} - lam
+//   twist       gamma_i = s_i + p_i + lam, r = argmin |gamma_i|
+//   vector      z_r = 1, z_{i+1} = -(ld_i / D-_{i+1}) z_i  (i >= r),
+//               z_i = -(ld_i / D+_i) z_{i+1}  (i < r)
+//   correction  lam += gamma_r / |z|^2   (Rayleigh quotient)
 __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
-  __shared__ double fd[TW_CH], fld[TW_CH], flld[TW_CH];  // ascending chunk
-  __shared__ double bd[TW_CH], bld[TW_CH], blld[TW_CH];  // descending chunk
+  __shared__ double qd_d[TW_CK], qd_ld[TW_CK], qd_lld[TW_CK];
+  __shared__ double buf[TW_CK][64];
   const int n = a.n;
   const int nbr = (n + 63) / 64;
   const int rep = ((int)blockIdx.x >= nbr) ? 1 : 0;
   const int lane = threadIdx.x;
   const int idx = ((int)blockIdx.x - rep * nbr) * 64 + lane;
   const bool live = idx < n;
-  const double* __restrict__ d = a.qd + (size_t)rep * 3 * n;
-  const double* __restrict__ ld = d + n;
-  const double* __restrict__ lld = d + 2 * (size_t)n;
+  const double* d = a.qd + (size_t)rep * 3 * n;
+  const double* ld = d + n;
+  const double* lld = d + 2 * (size_t)n;
   const int64_t T = a.T;
   const int col = rep * n + (live ? idx : 0);
-  double* __restrict__ Lp = a.Lp + col;
-  double* __restrict__ S = a.S + col;
-  double* __restrict__ Um = a.Um + col;
-  double* __restrict__ out = (rep ? a.Ut : a.Vt) + (live ? idx : 0);
+  double* Sck = a.Sck + col;
+  double* Pck = a.Pck + col;
+  double* out = (rep ? a.Ut : a.Vt) + (live ? idx : 0);
   const int64_t ldv = a.ldv;
   auto oidx = [&](int j) -> int64_t { return (int64_t)(rep ? (n - 1 - j) : j) * ldv; };
+  const int nck = (n + TW_CK - 1) / TW_CK;
+  auto stage = [&](int c0, int len) {
+    __syncthreads();
+    if (lane < len) {
+      qd_d[lane] = d[c0 + lane];
+      qd_ld[lane] = ld[c0 + lane];
+      qd_lld[lane] = lld[c0 + lane];
+    }
+    __syncthreads();
+  };
 
   const double smax = a.smax;
   const double pivmin = TW_TINY * fmax(1.0, smax * smax);
@@ -107,138 +126,145 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
   double hi = (sig + dl) * (sig + dl);
   const double dn1 = d[n - 1];
 
-  // Sturm counts at lo and hi in one ascending pass (two independent chains)
+  // Sturm counts at lo and hi (two independent chains)
   {
     double s1 = -lo, s2 = -hi;
     int c1 = 0, c2 = 0;
-    for (int c0 = 0; c0 < n - 1; c0 += TW_CH) {
-      const int len = min(TW_CH, n - 1 - c0);
-      __syncthreads();
-      for (int e = lane; e < len; e += 64) {
-        fd[e] = d[c0 + e];
-        flld[e] = lld[c0 + e];
-      }
-      __syncthreads();
+    for (int c = 0; c < nck; ++c) {
+      const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+      stage(c0, len);
       for (int e = 0; e < len; ++e) {
-        double p1 = guard(fd[e] + s1, pivmin);
-        double p2 = guard(fd[e] + s2, pivmin);
+        double p1 = guard(qd_d[e] + s1, pivmin);
+        double p2 = guard(qd_d[e] + s2, pivmin);
         c1 += (p1 < 0.0);
         c2 += (p2 < 0.0);
-        s1 = flld[e] * s1 / p1 - lo;
-        s2 = flld[e] * s2 / p2 - hi;
+        s1 = qd_lld[e] * s1 / p1 - lo;
+        s2 = qd_lld[e] * s2 / p2 - hi;
       }
     }
-    c1 += (guard(dn1 + s1, pivmin) < 0.0);
-    c2 += (guard(dn1 + s2, pivmin) < 0.0);
     if (c1 > idx) lo = 0.0;
     if (c2 < idx + 1) hi = (smax + dl) * (smax + dl) * (1.0 + 1e-8);
   }
 
   double lam = sig * sig;
-  double nz2 = 1.0, corr = 0.0;
+  double nz2 = 1.0, corr = 0.0, prev = INFINITY;
   bool active = live;
-  for (int it = 0; it < 24; ++it) {
+  for (int it = 0; it < 10; ++it) {
     if (__ballot(active) == 0) break;
-    // stationary (ascending, stores s_i and L+_i, counts negative pivots)
-    // and progressive (descending, stores p_i and U-_i) transforms in lockstep
-    double s = -lam;
-    double p = dn1 - lam;
+    // stationary transform with checkpoints and the Sturm count at lam
     int cnt = 0;
-    if (active) out[(int64_t)(n - 1) * ldv] = p;  // p_{n-1}
-    for (int c0 = 0; c0 < n - 1; c0 += TW_CH) {
-      const int len = min(TW_CH, n - 1 - c0);
-      const int g0 = n - 1 - c0 - len;  // descending chunk covers [g0, g0+len)
-      __syncthreads();
-      for (int e = lane; e < len; e += 64) {
-        fd[e] = d[c0 + e];
-        fld[e] = ld[c0 + e];
-        flld[e] = lld[c0 + e];
-        bd[e] = d[g0 + e];
-        bld[e] = ld[g0 + e];
-        blld[e] = lld[g0 + e];
-      }
-      __syncthreads();
-      if (active) {
+    {
+      double s = -lam;
+      for (int c = 0; c < nck; ++c) {
+        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+        stage(c0, len);
+        if (active) Sck[(int64_t)c * T] = s;
         for (int e = 0; e < len; ++e) {
-          const int f = c0 + e;
-          const int gi = len - 1 - e;  // descending: g = g0 + gi
-          const int64_t g = g0 + gi;
-          S[(int64_t)f * T] = s;
-          double dp = guard(fd[e] + s, pivmin);
+          double dp = guard(qd_d[e] + s, pivmin);
           cnt += (dp < 0.0);
-          Lp[(int64_t)f * T] = fld[e] / dp;
-          s = flld[e] * s / dp - lam;
-          double dm = guard(blld[gi] + p, pivmin);
-          Um[g * T] = bld[gi] / dm;
-          p = bd[gi] * p / dm - lam;
-          out[g * ldv] = p;  // p_g (scratch use of the output column)
+          s = qd_lld[e] * s / dp - lam;  // past the last index: unused
+        }
+      }
+    }
+    // progressive transform (descending) with checkpoints; the chunk's s
+    // values are recomputed into LDS to form gamma and its argmin
+    int r = n - 1;
+    double gbest = INFINITY;
+    {
+      double p = dn1 - lam;
+      for (int c = nck - 1; c >= 0; --c) {
+        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+        stage(c0, len);
+        double ss = Sck[(int64_t)c * T];
+        for (int e = 0; e < len; ++e) {
+          buf[e][lane] = ss;
+          double dp = guard(qd_d[e] + ss, pivmin);
+          ss = qd_lld[e] * ss / dp - lam;
+        }
+        for (int e = len - 1; e >= 0; --e) {
+          const int i = c0 + e;
+          if (i < n - 1) {
+            double dm = guard(qd_lld[e] + p, pivmin);
+            p = qd_d[e] * p / dm - lam;
+          }
+          const double g = buf[e][lane] + p + lam;
+          if (fabs(g) < fabs(gbest)) {
+            gbest = g;
+            r = i;
+          }
+        }
+        if (active) Pck[(int64_t)c * T] = p;  // p at the chunk's first index
+      }
+    }
+    // z downward from r: needs p_{i+1}, recomputed per chunk from the next
+    // chunk's checkpoint
+    if (active) {
+      nz2 = 1.0;  // converged lanes keep the norm of their final vector
+      out[oidx(r)] = 1.0;
+    }
+    {
+      double z = 1.0;
+      for (int c = 0; c < nck; ++c) {
+        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+        stage(c0, len);
+        double pp;
+        int etop;
+        if (c + 1 < nck) {
+          pp = Pck[(int64_t)(c + 1) * T];  // p_{c0+len}
+          etop = len - 1;
+        } else {
+          pp = dn1 - lam;  // p_{n-1}
+          etop = len - 2;
+        }
+        for (int e = etop; e >= 0; --e) {
+          buf[e][lane] = pp;  // p_{c0+e+1}
+          double dm = guard(qd_lld[e] + pp, pivmin);
+          pp = qd_d[e] * pp / dm - lam;
+        }
+        for (int e = 0; e <= etop; ++e) {
+          const int i = c0 + e;
+          if (active && i >= r) {
+            z = -(qd_ld[e] / guard(qd_lld[e] + buf[e][lane], pivmin)) * z;
+            nz2 += z * z;
+            out[oidx(i + 1)] = z;
+          }
+        }
+      }
+    }
+    // z upward from r: needs s_i, recomputed per chunk from its checkpoint
+    {
+      double z = 1.0;
+      for (int c = nck - 1; c >= 0; --c) {
+        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+        stage(c0, len);
+        double ss = Sck[(int64_t)c * T];
+        for (int e = 0; e < len; ++e) {
+          buf[e][lane] = ss;
+          double dp = guard(qd_d[e] + ss, pivmin);
+          ss = qd_lld[e] * ss / dp - lam;
+        }
+        for (int e = len - 1; e >= 0; --e) {
+          const int i = c0 + e;
+          if (active && i < r) {
+            z = -(qd_ld[e] / guard(qd_d[e] + buf[e][lane], pivmin)) * z;
+            nz2 += z * z;
+            out[oidx(i)] = z;
+          }
         }
       }
     }
     if (!active) continue;
-    S[(int64_t)(n - 1) * T] = s;
-    cnt += (guard(dn1 + s, pivmin) < 0.0);
-    // twist index: smallest |gamma_i| = |s_i + p_i + lam| (scan from the
-    // top; ties keep the larger index)
-    int r = n - 1;
-    double gbest = s + out[(int64_t)(n - 1) * ldv] + lam;
-    for (int i0 = n - 2; i0 >= 0; i0 -= 8) {
-      double sv[8], pv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = max(i0 - u, 0);
-        sv[u] = S[(int64_t)i * T];
-        pv[u] = out[(int64_t)i * ldv];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double gm = sv[u] + pv[u] + lam;
-        if (i0 - u >= 0 && fabs(gm) < fabs(gbest)) {
-          gbest = gm;
-          r = i0 - u;
-        }
-      }
-    }
-    // z with z_r = 1: downward z_{i+1} = -U-_i z_i (i >= r), upward
-    // z_i = -L+_i z_{i+1} (i < r); both sweeps run over the whole range so
-    // that the lanes stay at the same index
-    nz2 = 1.0;
-    double z = 1.0;
-    out[oidx(r)] = 1.0;
-    for (int i0 = 0; i0 < n - 1; i0 += 8) {
-      double uv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) uv[u] = Um[(int64_t)min(i0 + u, n - 2) * T];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = i0 + u;
-        if (i < n - 1 && i >= r) {
-          z = -uv[u] * z;
-          nz2 += z * z;
-          out[oidx(i + 1)] = z;
-        }
-      }
-    }
-    z = 1.0;
-    for (int i0 = n - 2; i0 >= 0; i0 -= 8) {
-      double lv[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) lv[u] = Lp[(int64_t)max(i0 - u, 0) * T];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = i0 - u;
-        if (i >= 0 && i < r) {
-          z = -lv[u] * z;
-          nz2 += z * z;
-          out[oidx(i)] = z;
-        }
-      }
-    }
     corr = gbest / nz2;
-    if (fabs(corr) <= 4.0 * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS)) {
+    // converged: correction at the rounding level of lam, the vector's norm
+    // saturated, or (from the third step on) the correction no longer
+    // halving -- it has reached the rounding floor of gamma_r, which for
+    // small singular values sits above eps*lam
+    if (fabs(corr) <= 4.0 * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS) ||
+        (it >= 2 && fabs(corr) > 0.5 * prev)) {
       active = false;
       continue;
     }
+    prev = fabs(corr);
     // cnt = number of eigenvalues below lam: keep the bracket around idx
     if (cnt <= idx) lo = lam; else hi = lam;
     double nl = lam + corr;
@@ -273,9 +299,11 @@ __global__ void tw_sign(int n, const double* a, const double* b, const double* V
     for (int j = 0; j < n; ++j) Ut[(int64_t)j * ldv + i] = -Ut[(int64_t)j * ldv + i];
 }
 
+static size_t tw_ck_size(int n) { return (size_t)((n + TW_CK - 1) / TW_CK) * 2 * n; }
+
 void twisted_plan(ArenaPlan& p, int n) {
   p.add<double>(6 * (size_t)n);
-  p.add<double>(3 * (size_t)n * 2 * n);
+  p.add<double>(2 * tw_ck_size(n));
   p.add<double>(1);
   p.add<double>(n);
 }
@@ -284,7 +312,7 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
                 double* Vt, double* Ut, int ldv, Arena& ws, hipStream_t st) {
   if (n < 1 || ldv < n) return DDC_E_ARG;
   double* qd = ws.take<double>(6 * (size_t)n);
-  double* scr = ws.take<double>(3 * (size_t)n * 2 * n);
+  double* scr = ws.take<double>(2 * tw_ck_size(n));
   double* hsmax = ws.take<double>(1);
   (void)hsmax;
   double* sig0 = ws.take<double>(n);
@@ -304,9 +332,8 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   t.Ut = Ut;
   t.ldv = ldv;
   t.T = 2 * n;
-  t.Lp = scr;
-  t.S = scr + (size_t)n * 2 * n;
-  t.Um = scr + 2 * (size_t)n * 2 * n;
+  t.Sck = scr;
+  t.Pck = scr + tw_ck_size(n);
   t.smax = smax;
   hipLaunchKernelGGL(tw_solve, dim3(2 * (unsigned)ceil_div(n, 64)), dim3(64), 0, st, t);
   DDC_LAUNCHED();
