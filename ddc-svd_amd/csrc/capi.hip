@@ -364,6 +364,41 @@ static int bdsvd_host(int n, const double* alpha, const double* beta, double* si
   return DDC_OK;
 }
 
+static int backward_dev(int m, int n, const double* U, int ldu, const double* sigma,
+                        const double* V, int ldv, const double* gU, int ldgu,
+                        const double* gs, const double* gV, int ldgv, double* gA, int ldga,
+                        double guard, hipStream_t user) {
+  DDC_TRY(check_dims(m, n));
+  const int mn = m < n ? m : n;
+  if (!U || !sigma || !V || !gA || ldu < m || ldv < n || ldga < m ||
+      (gU && ldgu < m) || (gV && ldgv < n) || !(guard >= 0.0)) {
+    set_err("invalid backward arguments");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = user ? user : g_ctx.st;
+  const bool wide = m < n;
+  const int mt = wide ? n : m;
+  ArenaPlan plan;
+  adjoint_plan(plan, mt, mn);
+  if (wide) plan.add<double>((size_t)mt * mn);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  if (!wide) {
+    DDC_TRY(adjoint_run(m, n, U, ldu, sigma, V, ldv, gU, ldgu, gs, gV, ldgv, gA, ldga, guard,
+                        ws, st));
+  } else {
+    // A^T = V S U^T is tall: its adjoint is Abar^T
+    double* tmp = ws.take<double>((size_t)mt * mn);
+    if (!tmp) return DDC_E_NOMEM;
+    DDC_TRY(adjoint_run(n, m, V, ldv, sigma, U, ldu, gV, ldgv, gs, gU, ldgu, tmp, n, guard,
+                        ws, st));
+    DDC_TRY(transpose_f64(n, m, tmp, n, gA, ldga, st));
+  }
+  DDC_HIP(hipStreamSynchronize(st));
+  return DDC_OK;
+}
+
 }  // namespace ddc
 
 using namespace ddc;
@@ -418,6 +453,16 @@ int bdsvd_vectors_gpu(int n, const double* alpha, const double* beta, double* si
     return DDC_E_ARG;
   }
   return bdsvd_host(n, alpha, beta, sigma, 1, X, Y);
+}
+
+int svd_gpu_backward(int m, int n, const double* U, int ldu, const double* sigma,
+                     const double* V, int ldv, const double* gU, int ldgu,
+                     const double* gsigma, const double* gV, int ldgv, double* gA, int ldga,
+                     double gap_guard, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return backward_dev(m, n, U, ldu, sigma, V, ldv, gU, ldgu, gsigma, gV, ldgv, gA, ldga,
+                      gap_guard, (hipStream_t)stream);
 }
 
 int svd_gpu_last_timings(double* ms, int count) {

@@ -101,6 +101,14 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
                   const double* Vt, int ldb, double* U, int ldu, double* V,
                   int ldv, Arena& ws, hipStream_t st);
 
+// Reverse-mode derivative of the thin SVD (m >= n): Abar from U, sigma, V and
+// the cotangents (null = zero).  Device pointers.
+void adjoint_plan(ArenaPlan& p, int m, int n);
+int adjoint_run(int m, int n, const double* U, int ldu, const double* sigma,
+                const double* V, int ldv, const double* Ub, int ldub, const double* sb,
+                const double* Vb, int ldvb, double* Ab, int ldab, double guard,
+                Arena& ws, hipStream_t st);
+
 // D (c x r) = S^T for the r x c column-major matrix S.
 int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd,
                   hipStream_t st);

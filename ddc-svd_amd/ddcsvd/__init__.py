@@ -68,6 +68,9 @@ def load():
         lib.bdsvd_values_gpu.restype = i
         lib.bdsvd_vectors_gpu.argtypes = [i, d, d, d, d, d]
         lib.bdsvd_vectors_gpu.restype = i
+        lib.svd_gpu_backward.argtypes = [i, i, v, i, v, v, i, v, i, v, v, i, v, i,
+                                         ctypes.c_double, v]
+        lib.svd_gpu_backward.restype = i
         lib.svd_gpu_last_timings.argtypes = [d, i]
         lib.svd_gpu_last_timings.restype = i
         lib.svd_gpu_last_error.argtypes = []
@@ -144,6 +147,51 @@ def svd_torch(A):
     stream = torch.cuda.current_stream(A.device).cuda_stream
     svd_dev(A, sigma, U, V, stream)
     return sigma, U, V
+
+
+def _colmajor_ld(t, name: str) -> int:
+    if t.dtype != _torch().float64 or not t.is_cuda or t.dim() != 2 or t.stride(0) != 1:
+        raise ValueError(f"{name} must be a column-major float64 CUDA matrix")
+    return max(t.stride(1), t.shape[0], 1)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def svd_backward_dev(U, sigma, V, gU, gsigma, gV, gA, gap_guard: float = 1e-13,
+                     stream=None) -> None:
+    """svd_gpu_backward on column-major float64 CUDA tensors.  gU, gsigma and gV
+    may be None (zero cotangent); gA (m x n) is written."""
+    m, mn = U.shape
+    n = V.shape[0]
+    ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    ldu, ldv, ldga = _colmajor_ld(U, "U"), _colmajor_ld(V, "V"), _colmajor_ld(gA, "gA")
+    ldgu = _colmajor_ld(gU, "gU") if gU is not None else 0
+    ldgv = _colmajor_ld(gV, "gV") if gV is not None else 0
+    if gsigma is not None and (not gsigma.is_contiguous() or gsigma.numel() != mn):
+        raise ValueError("gsigma must be a contiguous vector of min(m, n) values")
+    if tuple(gA.shape) != (m, n) or V.shape[1] != mn or sigma.numel() != mn:
+        raise ValueError("inconsistent shapes")
+    rc = load().svd_gpu_backward(m, n, ptr(U), ldu, ptr(sigma), ptr(V), ldv, ptr(gU), ldgu,
+                                 ptr(gsigma), ptr(gV), ldgv, ptr(gA), ldga, float(gap_guard),
+                                 ctypes.c_void_p(stream))
+    _check(rc)
+
+
+def svd_backward_torch(U, sigma, V, gU, gsigma, gV, gap_guard: float = 1e-13):
+    """dL/dA for A = U diag(sigma) V^T from CUDA float64 tensors; returns a
+    column-major tensor."""
+    torch = _torch()
+    m, n = U.shape[0], V.shape[0]
+    cm = lambda t: None if t is None else col_major(t.to(torch.float64))  # noqa: E731
+    gA = col_major(torch.empty((m, n), dtype=torch.float64, device=U.device))
+    gs = None if gsigma is None else gsigma.to(torch.float64).contiguous()
+    stream = torch.cuda.current_stream(U.device).cuda_stream
+    svd_backward_dev(cm(U), sigma.contiguous(), cm(V), cm(gU), gs, cm(gV), gA, gap_guard,
+                     stream)
+    return gA
 
 
 def bidiag(A: np.ndarray):

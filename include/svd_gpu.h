@@ -87,6 +87,22 @@ int bdsvd_values_gpu(int n, const double *alpha, const double *beta,
 int bdsvd_vectors_gpu(int n, const double *alpha, const double *beta,
                       double *sigma, double *X, double *Y);
 
+/*
+ * Reverse-mode derivative of the thin SVD (no reference counterpart: the
+ * reference has no derivative).  Given A = U diag(sigma) V^T as returned by
+ * svd_gpu_dev (U m x min(m,n), V n x min(m,n), sigma ascending) and the
+ * cotangents gU, gsigma, gV of a scalar loss (each may be NULL = zero),
+ * writes gA = dL/dA (m x n).  Device pointers, column-major.
+ *   gap_guard  regularises 1/(sigma_j^2 - sigma_i^2) for near-equal values:
+ *              below delta = gap_guard * sigma_max^2 the factor is d/delta^2
+ *              (1e-13 is a good default; 0 = unguarded)
+ * The formula is exact for distinct nonzero singular values.
+ */
+int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma,
+                     const double *V, int ldv, const double *gU, int ldgu,
+                     const double *gsigma, const double *gV, int ldgv, double *gA,
+                     int ldga, double gap_guard, void *stream);
+
 /* Milliseconds of the stages of the last successful call on this thread's
  * process: [0] copy-in, [1] bidiagonalization, [2] singular values,
  * [3] vectors, [4] back-transformation, [5] copy-out.  Returns the number of
