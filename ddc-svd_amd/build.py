@@ -33,7 +33,7 @@ FLAGS = [
 ]
 
 SOURCES = ["gemm_f64.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
-           "adjoint.hip", "capi.hip"]
+           "adjoint.hip", "batched.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -35,6 +35,9 @@ void set_err(const char* fmt, ...) {
 }
 }  // namespace
 
+void report_error(const char* msg) { set_err("%s", msg); }
+void clear_error() { g_err[0] = 0; }
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line) {
   set_err("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
   return DDC_E_HIP;

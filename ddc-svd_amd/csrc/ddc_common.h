@@ -24,6 +24,9 @@ namespace ddc {
 // Records the first HIP failure of the calling thread so entry points can
 // report it; returns DDC_E_HIP on failure.
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+// Message returned by svd_gpu_last_error(); clear_error() empties it.
+void report_error(const char* msg);
+void clear_error();
 
 #define DDC_HIP(call)                                                     \
   do {                                                                    \

@@ -103,6 +103,61 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
                      const double *gsigma, const double *gV, int ldgv, double *gA,
                      int ldga, double gap_guard, void *stream);
 
+/*
+ * Batched SVD of many small matrices of uneven shapes (no reference
+ * counterpart: the reference factors one matrix per call, svd_gpu.c:53).
+ * One persistent launch, replayed from a captured HIP graph, factors every
+ * matrix with one-sided Jacobi; 1 <= m, n <= 256 per matrix.
+ *
+ * Buffers are packed back to back in batch order, column-major, on the
+ * device; svd_gpu_batched_offsets gives the element offsets (count + 1
+ * entries each, the last one the total):
+ *   A_k      m_k x n_k        at dA + offA[k]
+ *   sigma_k  min(m_k, n_k)    at dsigma + offS[k], ascending
+ *   U_k      m_k x min(m_k,n_k) at dU + offU[k]
+ *   V_k      n_k x min(m_k,n_k) at dV + offV[k]
+ * dtype selects double (SVD_GPU_F64) or float (SVD_GPU_F32) buffers.
+ * A is not modified.  Columns of U (or of V for a wide matrix) that belong
+ * to an exactly zero singular value are completed to an orthonormal set.
+ */
+#define SVD_GPU_F64 0
+#define SVD_GPU_F32 1
+typedef struct svd_gpu_batch svd_gpu_batch;
+
+/* Describes the batch; allocates the device descriptors and workspace. */
+int svd_gpu_batched_plan(svd_gpu_batch **plan, int count, const int *m,
+                         const int *n, int dtype);
+/* Element offsets of the packed buffers (any pointer may be NULL). */
+int svd_gpu_batched_offsets(const svd_gpu_batch *plan, long long *offA,
+                            long long *offS, long long *offU, long long *offV);
+/*
+ * Factors the batch, ordered on stream (NULL = the null stream); returns once
+ * the work is enqueued.  dinfo (device, count ints, may be NULL) receives the
+ * Jacobi sweeps each matrix took, or -1 if it did not converge.  The first
+ * call autotunes the launch (if svd_gpu_batched_tune was not called) and
+ * captures a graph; later calls with the same buffers replay it and allocate
+ * nothing.  Calls on one plan must not overlap.
+ */
+int svd_gpu_batched_run(svd_gpu_batch *plan, const void *dA, void *dsigma,
+                        void *dU, void *dV, int *dinfo, void *stream);
+/* Times the kernel variants, launch sizes and cooperation thresholds on half
+ * of the batch (the result is also remembered for later plans of similar
+ * batches); writes the outputs of that half.  Synchronous. */
+int svd_gpu_batched_tune(svd_gpu_batch *plan, const void *dA, void *dsigma,
+                         void *dU, void *dV, void *stream);
+/* Chosen configuration: kernel variant, resident 4-wave workgroups per CU,
+ * the smallest min(m, n) that a whole workgroup factors together, and the
+ * column-block widths for matrices of up to 128 / up to 256 rows.  Returns
+ * the number of variants (variant = -1 before tuning). */
+int svd_gpu_batched_config(const svd_gpu_batch *plan, int *variant,
+                           int *wgs_per_cu, int *coop_min_n, int *block_small,
+                           int *block_large);
+/* Forces a configuration (variant < number of variants, 1..8 workgroups per
+ * CU, coop_min_n >= 1; 257 disables cooperation). */
+int svd_gpu_batched_set_config(svd_gpu_batch *plan, int variant, int wgs_per_cu,
+                               int coop_min_n);
+void svd_gpu_batched_destroy(svd_gpu_batch *plan);
+
 /* Milliseconds of the stages of the last successful call on this thread's
  * process: [0] copy-in, [1] bidiagonalization, [2] singular values,
  * [3] vectors, [4] back-transformation, [5] copy-out.  Returns the number of
