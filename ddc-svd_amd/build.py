@@ -22,7 +22,8 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libddcsvd.so")
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 ARCH = os.environ.get("DDCSVD_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -32,7 +33,7 @@ FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["gemm_f64.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
+SOURCES = ["blas.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
            "adjoint.hip", "batched.hip", "capi.hip"]
 
 
@@ -54,7 +55,8 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs,
+           "-L" + os.path.join(ROCM, "lib"), "-lrocblas"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

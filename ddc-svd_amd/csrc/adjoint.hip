@@ -10,8 +10,8 @@
 //
 // Rewritten for the GPU as  Abar = (U W + Ubar S^-1) V^T  with
 //   W = (J o F) S + S (K o F) + diag(sbar) - (U^T Ubar) S^-1,
-// which costs three MFMA GEMMs of the thin size (U^T Ubar, V^T Vbar and the
-// two products) plus one elementwise "F-matrix" kernel.  The gap guard
+// which costs four library GEMMs of the thin size (U^T Ubar, V^T Vbar and the
+// two products, FP64 MFMA) plus one elementwise "F-matrix" kernel.  The gap guard
 // replaces 1/d by d/delta^2 when |d| = |s_j^2 - s_i^2| < delta =
 // guard * s_max^2 (continuous at |d| = delta; guard 0 gives the plain
 // formula), and 1/s_j by 0 for s_j <= guard * s_max.
@@ -85,10 +85,8 @@ __global__ void adj_scale(int m, int n, const double* Ub, int ldub, const double
 }
 
 void adjoint_plan(ArenaPlan& p, int m, int n) {
-  p.add<double>((size_t)n * n * 3);   // P, Q, W
-  p.add<double>((size_t)m * n);       // G
-  p.add<double>((size_t)16 * n * n);  // split-K partials
-  p.add<double>(1);
+  p.add<double>((size_t)n * n * 3);  // P, Q, W
+  p.add<double>((size_t)m * n);      // G
 }
 
 // Tall case (m >= n).  All pointers on the device; null cotangents are zero.
@@ -101,21 +99,13 @@ int adjoint_run(int m, int n, const double* U, int ldu, const double* sigma,
   double* Q = P + (size_t)n * n;
   double* W = Q + (size_t)n * n;
   double* G = ws.take<double>((size_t)m * n);
-  const size_t part_cap = (size_t)16 * n * n;
-  double* part = ws.take<double>(part_cap);
-  if (!part) return DDC_E_NOMEM;
+  if (!G) return DDC_E_NOMEM;
   double smax = 0.0;
   DDC_HIP(hipMemcpyAsync(&smax, sigma + (n - 1), sizeof(double), hipMemcpyDeviceToHost, st));
   DDC_HIP(hipStreamSynchronize(st));
 
-  if (Ub)
-    DDC_TRY(gemm_f64_split(n, n, m, 1.0, GemmOperand{U, ldu, 'T', MASK_NONE, 0},
-                           GemmOperand{Ub, ldub, 'N', MASK_NONE, 0}, 0.0, P, n, part,
-                           part_cap, st));
-  if (Vb)
-    DDC_TRY(gemm_f64_split(n, n, n, 1.0, GemmOperand{V, ldv, 'T', MASK_NONE, 0},
-                           GemmOperand{Vb, ldvb, 'N', MASK_NONE, 0}, 0.0, Q, n, part,
-                           part_cap, st));
+  if (Ub) DDC_TRY(dgemm('T', 'N', n, n, m, 1.0, U, ldu, Ub, ldub, 0.0, P, n, st));
+  if (Vb) DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, V, ldv, Vb, ldvb, 0.0, Q, n, st));
   AdjArgs a;
   a.n = n;
   a.sigma = sigma;
@@ -133,10 +123,8 @@ int adjoint_run(int m, int n, const double* U, int ldu, const double* sigma,
                      dim3(256), 0, st, m, n, Ub, ldub, sigma, a.smin, G, m);
   DDC_LAUNCHED();
   // G += U W ; Abar = G V^T
-  DDC_TRY(gemm_f64(m, n, n, 1.0, GemmOperand{U, ldu, 'N', MASK_NONE, 0},
-                   GemmOperand{W, n, 'N', MASK_NONE, 0}, 1.0, G, m, st));
-  DDC_TRY(gemm_f64(m, n, n, 1.0, GemmOperand{G, m, 'N', MASK_NONE, 0},
-                   GemmOperand{V, ldv, 'T', MASK_NONE, 0}, 0.0, Ab, ldab, st));
+  DDC_TRY(dgemm('N', 'N', m, n, n, 1.0, U, ldu, W, n, 1.0, G, m, st));
+  DDC_TRY(dgemm('N', 'T', m, n, n, 1.0, G, m, V, ldv, 0.0, Ab, ldab, st));
   return DDC_OK;
 }
 

@@ -8,8 +8,10 @@
 // for the unit-norm convention H = I - 2 w w^T (tau = 2).  T is never formed:
 // its inverse is T^{-1} = I/2 + strict_upper(W^T W) (the identity behind
 // the recursive T-factor of LAPACK dlarft), so a block costs
-//     Z = W^T C  (MFMA GEMM),  Z = T Z by one upper-triangular solve per
-//     column of Z,  C -= W Z  (MFMA GEMM).
+//     G = W^T W,  Z = W^T C,  Z = T Z (triangular solve with T^{-1}),
+//     C -= W Z
+// with the block's reflectors first packed into a dense zero-padded W; the
+// three products are library GEMMs (rocBLAS, FP64 MFMA).
 // Left reflectors are the columns A[k:m, k]; right reflectors are the rows
 // A[k, k+1:n] (leading dimension n for V -- the reference indexes the
 // transposed copy with m, bidiag_par.c:1034-1036, wrong when m != n).
@@ -19,7 +21,7 @@
 
 namespace ddc {
 
-constexpr int NBB = 64;
+constexpr int NBB = 128;  // reflectors per compact-WY block
 
 // C(0:n, 0:n) = Bt^T where Bt[j*ldb + i] holds vector i, entry j;
 // rows n..rows-1 of C are zeroed.
@@ -63,78 +65,76 @@ int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd, hi
   return DDC_OK;
 }
 
-// Z <- T Z with T^{-1} = I/2 + strict_upper(G), G = W^T W (nb x nb, ld nb):
-// back substitution z_i = 2 (z_i - sum_{j>i} G(i,j) z_j), one thread per
-// column of Z (ld nb).
-__global__ __launch_bounds__(64) void bt_trsm(int nb, int ncol, const double* G, double* Z) {
-  __shared__ double Gs[NBB * NBB];
-  for (int e = threadIdx.x; e < nb * nb; e += 64) Gs[e] = G[e];
-  __syncthreads();
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  if (c >= ncol) return;
-  double z[NBB];
-  double* zc = Z + (int64_t)c * nb;
-#pragma unroll
-  for (int i = 0; i < NBB; ++i) z[i] = (i < nb) ? zc[i] : 0.0;
-#pragma unroll
-  for (int i = NBB - 1; i >= 0; --i) {
-    if (i >= nb) continue;
-    double acc = z[i];
-#pragma unroll
-    for (int j = i + 1; j < NBB; ++j)
-      if (j < nb) acc -= Gs[i + j * nb] * z[j];
-    z[i] = 2.0 * acc;
+// Dense copy Wd (rows x nb, ld ldw) of a block of reflectors, zeros
+// outside their support.  left: Wd(i, c) = A[k0+i, k0+c] for i >= c.
+// right: Wd(i, c) = A[k0+c, k0+1+i] for i >= c (rows of A, read through an
+// LDS tile so that both sides stay coalesced).
+template <bool RIGHT>
+__global__ void bt_pack(int rows, int nb, const double* S, int lda, double* Wd, int ldw) {
+  __shared__ double tile[32][33];
+  const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  if (!RIGHT) {
+    for (int q = ty; q < 32; q += 8) {
+      const int i = i0 + tx, c = c0 + q;
+      if (i < rows && c < nb) Wd[i + (int64_t)c * ldw] = i >= c ? S[i + (int64_t)c * lda] : 0.0;
+    }
+  } else {
+    for (int q = ty; q < 32; q += 8) {  // tile[c][i] = S(c, i), coalesced over c
+      const int c = c0 + tx, i = i0 + q;
+      tile[q][tx] = (c < nb && i < rows && i >= c) ? S[c + (int64_t)i * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int q = ty; q < 32; q += 8) {
+      const int i = i0 + tx, c = c0 + q;
+      if (i < rows && c < nb) Wd[i + (int64_t)c * ldw] = tile[tx][q];
+    }
   }
-#pragma unroll
-  for (int i = 0; i < NBB; ++i)
-    if (i < nb) zc[i] = z[i];
 }
 
-constexpr int SPLITS = 16;
+// T^{-1} = I/2 + strict_upper(G) (nb x nb, ld nb) for the block
+// H_0 ... H_{nb-1} = I - W T W^T of unit-norm reflectors (tau = 2).
+__global__ void bt_tinv(int nb, const double* G, double* Ti) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= nb * nb) return;
+  const int i = e % nb, j = e / nb;
+  Ti[e] = i < j ? G[e] : (i == j ? 0.5 : 0.0);
+}
 
 void backtrans_plan(ArenaPlan& p, int m, int n) {
-  p.add<double>((size_t)NBB * NBB * 2);        // G, T
-  p.add<double>((size_t)NBB * n * 2);          // Z, Z2
-  p.add<double>((size_t)SPLITS * NBB * n);     // split-K partials
+  p.add<double>((size_t)NBB * NBB * 2);  // G, T^{-1}
+  p.add<double>((size_t)NBB * n);        // Z
+  p.add<double>((size_t)m * NBB);        // packed reflectors
 }
 
-// Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C.
+// Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C:
+// Z = W^T C, Z = T Z (triangular solve with T^{-1}), C -= W Z.
 // left: w_k = A[k:m, k];  right: w_k = A[k, k+1:n] acting on rows k+1..n-1.
-static int apply_blocks(bool left, int m, int n, const double* A, int lda,
-                        int nref, double* C, int ldc, int ncol, double* G,
-                        double* T, double* Z, double* Z2, double* part,
-                        size_t part_cap, hipStream_t st) {
+static int apply_blocks(bool left, int m, int n, const double* A, int lda, int nref, double* C,
+                        int ldc, int ncol, double* G, double* Ti, double* Z, double* Wd,
+                        hipStream_t st) {
   const int nblk = (int)ceil_div(nref, NBB);
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int k0 = bi * NBB;
     const int nb = std::min(NBB, nref - k0);
-    GemmOperand W, Wt;
-    int rows;
-    double* Csub;
-    if (left) {
-      // W = A[k0:m, k0:k0+nb], w_c has support rows >= c (local)
-      rows = m - k0;
-      W = GemmOperand{A + k0 + (int64_t)k0 * lda, lda, 'N', MASK_LOWER, 0};
-      Csub = C + k0;
-    } else {
-      // R = A[k0:k0+nb, k0+1:n] (row storage), W = R^T; w_c has support
-      // local columns >= c
-      rows = n - k0 - 1;
-      W = GemmOperand{A + k0 + (int64_t)(k0 + 1) * lda, lda, 'T', MASK_UPPER1, 0};
-      Csub = C + k0 + 1;
-    }
+    const int rows = left ? m - k0 : n - k0 - 1;
     if (rows <= 0) continue;
-    Wt = W;
-    Wt.op = (W.op == 'N') ? 'T' : 'N';
-    // G = W^T W ; Z = W^T C ; Z = T Z ; C -= W Z
-    DDC_TRY(gemm_f64_split(nb, nb, rows, 1.0, Wt, W, 0.0, G, nb, part, part_cap, st));
-    GemmOperand Cop{Csub, ldc, 'N', MASK_NONE, 0};
-    DDC_TRY(gemm_f64_split(nb, ncol, rows, 1.0, Wt, Cop, 0.0, Z, nb, part, part_cap, st));
-    hipLaunchKernelGGL(bt_trsm, dim3((unsigned)ceil_div(ncol, 64)), dim3(64), 0, st, nb, ncol,
-                       G, Z);
+    double* Csub = left ? C + k0 : C + k0 + 1;
+    dim3 gp((unsigned)ceil_div(rows, 32), (unsigned)ceil_div(nb, 32));
+    if (left)
+      hipLaunchKernelGGL(bt_pack<false>, gp, dim3(32, 8), 0, st, rows, nb,
+                         A + k0 + (int64_t)k0 * lda, lda, Wd, rows);
+    else
+      hipLaunchKernelGGL(bt_pack<true>, gp, dim3(32, 8), 0, st, rows, nb,
+                         A + k0 + (int64_t)(k0 + 1) * lda, lda, Wd, rows);
     DDC_LAUNCHED();
-    GemmOperand Z2op{Z, nb, 'N', MASK_NONE, 0};
-    DDC_TRY(gemm_f64(rows, ncol, nb, -1.0, W, Z2op, 1.0, Csub, ldc, st));
+    DDC_TRY(dgemm('T', 'N', nb, nb, rows, 1.0, Wd, rows, Wd, rows, 0.0, G, nb, st));
+    hipLaunchKernelGGL(bt_tinv, dim3((unsigned)ceil_div(nb * nb, 256)), dim3(256), 0, st, nb, G,
+                       Ti);
+    DDC_LAUNCHED();
+    DDC_TRY(dgemm('T', 'N', nb, ncol, rows, 1.0, Wd, rows, Csub, ldc, 0.0, Z, nb, st));
+    DDC_TRY(dtrsm_left_upper(nb, ncol, Ti, nb, Z, nb, st));
+    DDC_TRY(dgemm('N', 'N', rows, ncol, nb, -1.0, Wd, rows, Z, nb, 1.0, Csub, ldc, st));
   }
   return DDC_OK;
 }
@@ -144,12 +144,10 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
                   int ldv, Arena& ws, hipStream_t st) {
   if (m < n || n < 1) return DDC_E_ARG;
   double* G = ws.take<double>((size_t)NBB * NBB * 2);
-  double* T = G + NBB * NBB;
-  double* Z = ws.take<double>((size_t)NBB * n * 2);
-  double* Z2 = Z + (size_t)NBB * n;
-  const size_t part_cap = (size_t)SPLITS * NBB * n;
-  double* part = ws.take<double>(part_cap);
-  if (!part) return DDC_E_NOMEM;
+  double* Ti = G + NBB * NBB;
+  double* Z = ws.take<double>((size_t)NBB * n);
+  double* Wd = ws.take<double>((size_t)m * NBB);
+  if (!Wd) return DDC_E_NOMEM;
   dim3 tb(32, 8);
   dim3 gu((unsigned)ceil_div(n, 32), (unsigned)ceil_div(m, 32));
   hipLaunchKernelGGL(bt_unpack, gu, tb, 0, st, n, m, Ut, ldb, U, ldu);
@@ -158,10 +156,9 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   hipLaunchKernelGGL(bt_unpack, gv, tb, 0, st, n, n, Vt, ldb, V, ldv);
   DDC_LAUNCHED();
   // left: reflectors 0..n-1 (the last one is the 1 x 1 sign flip when m == n)
-  DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, T, Z, Z2, part, part_cap, st));
+  DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st));
   // right: reflectors 0..n-3 (row n-2 holds no reflection)
-  if (n > 2)
-    DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, T, Z, Z2, part, part_cap, st));
+  if (n > 2) DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, Ti, Z, Wd, st));
   return DDC_OK;
 }
 

@@ -21,8 +21,8 @@
 // tiny fixed-order reduction, so no grid-wide synchronisation is needed and
 // results are bitwise reproducible).  The two GEMVs are the only passes
 // over the trailing matrix: 16 bytes per trailing element per step.  After
-// NB steps the trailing matrix gets A0 -= U Y^T + X V^T on the FP64 matrix
-// cores (gemm_f64_2), one read and one write.
+// NB steps the trailing matrix gets A0 -= [U X] [Y V]^T, one FP64 MFMA GEMM
+// of inner dimension 2 NB (rocBLAS), one read and one write.
 #include <algorithm>
 
 #include "ddc_common.h"
@@ -479,7 +479,7 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
 void bidiag_plan(ArenaPlan& p, int m, int n, int lda) {
   BidiagBufs d = bidiag_dims(m, n, lda);
   p.add<double>((size_t)d.ldy * NB * 2);   // Y, V
-  p.add<double>((size_t)lda * NB);         // X
+  p.add<double>((size_t)lda * NB * 2);     // copy of the panel's U, then X
   p.add<double>(lda);                      // cvec
   p.add<double>(d.ldy);                    // rvec
   p.add<double>((size_t)d.nchL_max * d.ldy);
@@ -504,7 +504,10 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
   a.beta = beta;
   a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
   a.V = a.Y + (size_t)d.ldy * NB;
-  a.X = ws.take<double>((size_t)lda * NB);
+  // [U X]: the panel's left reflectors are copied next to X before the
+  // trailing update, so that [U X] and [Y V] are single dense operands
+  double* UX = ws.take<double>((size_t)lda * NB * 2);
+  a.X = UX ? UX + (size_t)lda * NB : nullptr;
   a.cvec = ws.take<double>(lda);
   a.rvec = ws.take<double>(d.ldy);
   a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
@@ -563,12 +566,11 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
     // blocked trailing update at the end of a full panel
     if (a.t == NB - 1 && a.p + NB < n) {
       const int r0 = a.p + NB;
-      GemmOperand U1{A + r0 + (int64_t)a.p * lda, lda, 'N', MASK_NONE, 0};
-      GemmOperand Y1{a.Y + r0, d.ldy, 'T', MASK_NONE, 0};
-      GemmOperand X2{a.X + r0, lda, 'N', MASK_NONE, 0};
-      GemmOperand V2{a.V + r0, d.ldy, 'T', MASK_NONE, 0};
-      DDC_TRY(gemm_f64_2(m - r0, n - r0, -1.0, U1, Y1, NB, X2, V2, NB, 1.0,
-                         A + r0 + (int64_t)r0 * lda, lda, st));
+      DDC_HIP(hipMemcpy2DAsync(UX + r0, sizeof(double) * lda, A + r0 + (int64_t)a.p * lda,
+                               sizeof(double) * lda, sizeof(double) * (m - r0), NB,
+                               hipMemcpyDeviceToDevice, st));
+      DDC_TRY(dgemm('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0, d.ldy, 1.0,
+                    A + r0 + (int64_t)r0 * lda, lda, st));
     }
   }
   return DDC_OK;

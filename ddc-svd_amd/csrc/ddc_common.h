@@ -117,32 +117,18 @@ int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd,
                   hipStream_t st);
 
 // ---------------------------------------------------------------------------
-// FP64 MFMA GEMM: C[M x N] = beta*C + alpha * op(A) * op(B), column-major.
-// op: 'N' or 'T'.  The mask selects a structured operand; it applies to the
-// STORED element X[i + j*ld], before op:
-//   MASK_NONE:   plain
-//   MASK_LOWER:  X(i, j) = 0 unless i >= j + moff  (left reflector panel)
-//   MASK_UPPER1: X(i, j) = 0 unless j >= i + moff  (right reflectors, rows)
+// Library GEMMs (rocBLAS, MFMA), column-major, op 'N' or 'T'.  Stream-ordered.
 // ---------------------------------------------------------------------------
-enum { MASK_NONE = 0, MASK_LOWER = 1, MASK_UPPER1 = 2 };
-struct GemmOperand {
-  const double* p;
-  int ld;
-  char op;     // 'N' or 'T'
-  int mask;    // MASK_*
-  int moff;    // mask offset
-};
-int gemm_f64(int M, int N, int K, double alpha, GemmOperand A, GemmOperand B,
-             double beta, double* C, int ldc, hipStream_t st);
-// The same, split over K when the output has few tiles: `part` (capacity
-// part_cap doubles) holds the partial products, summed in a fixed order.
-int gemm_f64_split(int M, int N, int K, double alpha, GemmOperand A, GemmOperand B,
-                   double beta, double* C, int ldc, double* part, size_t part_cap,
-                   hipStream_t st);
-// Two operand pairs accumulated into one pass over C.
-int gemm_f64_2(int M, int N, double alpha, GemmOperand A1, GemmOperand B1,
-               int K1, GemmOperand A2, GemmOperand B2, int K2, double beta,
-               double* C, int ldc, hipStream_t st);
+int dgemm(char ta, char tb, int M, int N, int K, double alpha, const double* A, int lda,
+          const double* B, int ldb, double beta, double* C, int ldc, hipStream_t st);
+int sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,
+          const float* B, int ldb, float beta, float* C, int ldc, hipStream_t st);
+// BF16 operands (raw 16-bit storage), FP32 accumulation and C.
+int gemm_bf16_f32(char ta, char tb, int M, int N, int K, float alpha, const void* A, int lda,
+                  const void* B, int ldb, float beta, float* C, int ldc, hipStream_t st);
+// B <- T^{-1} B for the M x M upper triangular T (non-unit diagonal).
+int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
+                     hipStream_t st);
 
 // Deterministic wave64 sum (fixed butterfly order).
 __device__ __forceinline__ double wave_sum(double v) {
