@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Headline benchmark: plain FP64 svd_gpu on one MI355X.
+"""Headline benchmark: plain FP64 svd_gpu on one MI355X, plus the batched
+and differentiation forms measured in the same run.
 
 A step is one full SVD (bidiagonalization, divide-and-conquer values,
 twisted-factorization vectors, back-transformation) of a dense m x n FP64
@@ -7,15 +8,22 @@ matrix with i.i.d. N(0,1) entries (seed 2) through the device-pointer entry
 point svd_gpu_dev, input already resident in HBM.  Default: 8192 x 8192.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--m 8192 --n 8192]
+                    [--no-forms] [--no-cpu] [--no-check]
 
-Prints one JSON line (rank 0).  The roofline object covers the
-bidiagonalization, whose algorithmic traffic is 16 S(m,n) bytes,
-S = sum_{i<n} (m-i)(n-i) (two FP64 reads of the trailing matrix per step);
-achieved = 16 S / t_bidiag with t_bidiag from HIP events around the stage on
-the library's stream.  The cpu_baseline times the serial bidiagonalization of
-the reference host path (the oracle restatement, bit-identical to the
-reference's bidiag_seq) on the first few steps of the same matrix and scales
-by the algorithmic work ratio.
+Prints one JSON line.  `roofline` covers the dominant kernel, the left GEMV
+of the bidiagonalization (bd_left_gemv): its algorithmic traffic is one FP64
+read of the trailing block A[k:m, k+1:n] per launch, 8 (m-k)(n-k-1) bytes;
+achieved = sum of those bytes / sum of the launch durations, both over the
+timed steps, the durations from HIP events recorded around every launch on
+the library's stream (svd_gpu_profile_kernels).  `traffic` is the measured
+HBM bytes per launch from the committed rocprofv3 counter summary
+(profiles/bd_left_gemv_8192x8192_traffic.json) when it matches the workload.
+`cpu_baseline` times the serial bidiagonalization of the reference host path
+(the oracle restatement, bit-identical to the reference's bidiag_seq) on the
+first steps of the same matrix and scales by the algorithmic work ratio.
+`forms` holds the batched SVD (10,000 matrices, m ~ U{16..256}, n ~ U{8..m},
+entries U(-1,1)) in FP64 and FP32 against a loop of plain svd_gpu_dev calls,
+and the forward + backward throughput of the differentiable SVD at 2048 x 1024.
 """
 from __future__ import annotations
 
@@ -31,6 +39,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ddc-svd_amd"))
 
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One progress line on stderr per finished phase."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 METRIC = ("SVD/s (8192² + 10k batch) vs plain FP64 at stated residual; "
           "bidiag HBM GB/s")
 
@@ -85,8 +99,217 @@ def cpu_baseline(m: int, n: int, seed: int, steps: int) -> dict:
     }
 
 
+def left_gemv_bytes(m: int, n: int) -> int:
+    """Algorithmic bytes of all left GEMV launches: sum_{k<n-1} 8 (m-k)(n-k-1)."""
+    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 1))
+
+
+def right_gemv_bytes(m: int, n: int) -> int:
+    """Algorithmic bytes of all right GEMV launches: sum_{k<n-2} 8 (m-k-1)(n-k-1)."""
+    return sum(8 * (m - k - 1) * (n - k - 1) for k in range(n - 2))
+
+
+def traffic_summary(m: int, n: int):
+    path = os.path.join(ROOT, "profiles", f"bd_left_gemv_{m}x{n}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
+def batched_form(seed: int = 0, count: int = 10000, reps: int = 3, loop_sample: int = 100,
+                 check_sample: int = 200) -> dict:
+    """10,000-matrix batch through one graph-replayed persistent launch,
+    FP64 and FP32, against a loop of plain svd_gpu_dev calls."""
+    import numpy as np
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+    from ddcsvd.batched import BatchedSvd
+
+    ms, ns = gen.batch_shapes(count, seed)
+    res = {"config": f"{count} matrices, m ~ U{{16..256}}, n ~ U{{8..m}}, entries U(-1,1), "
+                     f"seed {seed}; one persistent launch per run, replayed from a HIP graph",
+           "sum_mn": int(np.sum(ms * ns))}
+    total = int(np.sum(ms * ns))
+    flat = gen.uniform01(total, seed, stream=11) * 2.0 - 1.0   # matrix k = its slice, col-major
+    progress("batched: generated the batch")
+    rng = np.random.default_rng(seed)
+    sample = np.sort(rng.choice(count, size=min(check_sample, count), replace=False))
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        b = BatchedSvd(ms, ns, dt)
+        A = torch.from_numpy(flat).to(device="cuda", dtype=dt)
+        S, U, V = b.outputs()
+        info = torch.zeros(count, dtype=torch.int32, device="cuda")
+        t0 = time.perf_counter()
+        b.run(A, S, U, V, info)           # tunes, captures the graph, runs
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        times = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.run(A, S, U, V, info)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        best = min(times)
+        worst_res, worst_sig = 0.0, 0.0
+        for k in sample:
+            a = flat[b.offA[k]:b.offA[k + 1]].reshape(ns[k], ms[k]).T
+            s, u, v = b.unpack(int(k), S, U, V)
+            sl = np.sort(np.linalg.svd(a, compute_uv=False))
+            worst_sig = max(worst_sig, float(np.max(np.abs(s - sl)) / sl[-1]))
+            worst_res = max(worst_res, float(np.linalg.norm(a - (u * s) @ v.T)
+                                             / np.linalg.norm(a)))
+        inf = info.cpu().numpy()
+        progress(f"batched {name}: {1e3 * best:.1f} ms per batch (first run with tuning "
+                 f"{first:.1f} s)")
+        res[name] = {"svd_per_s": count / best, "ms": 1e3 * best,
+                     "ms_all_reps": [1e3 * t for t in times],
+                     "first_run_s_incl_tuning": first, "tuned": b.config(),
+                     "sweeps_mean": float(inf.mean()), "unconverged": int(np.sum(inf < 0)),
+                     "checked_matrices": int(len(sample)),
+                     "worst_residual": worst_res, "worst_sigma_err_rel_max": worst_sig}
+        b.close()
+        del A, S, U, V
+    # loop of plain svd_gpu_dev calls over the first matrices of the batch
+    mats = []
+    offs = np.concatenate([[0], np.cumsum(ms * ns)])
+    for k in range(loop_sample):
+        a = flat[offs[k]:offs[k + 1]].reshape(ns[k], ms[k]).T
+        mats.append(ddcsvd.col_major(torch.from_numpy(np.ascontiguousarray(a)).cuda()))
+    ddcsvd.svd_torch(mats[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a in mats:
+        ddcsvd.svd_torch(a)
+    torch.cuda.synchronize()
+    dt_loop = time.perf_counter() - t0
+    res["plain_loop"] = {"svd_per_s": loop_sample / dt_loop,
+                         "sample": f"first {loop_sample} matrices of the batch, one "
+                                   f"svd_gpu_dev call each"}
+    res["speedup_fp64_vs_plain_loop"] = res["fp64"]["svd_per_s"] / res["plain_loop"]["svd_per_s"]
+    return res
+
+
+def backward_form(reps: int = 3) -> dict:
+    """Forward + backward of the differentiable SVD at 2048 x 1024."""
+    import numpy as np
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+
+    m, n = 2048, 1024
+    A = ddcsvd.col_major(torch.from_numpy(
+        gen.graded_matrix(m, n, np.linspace(1.0, 2.0, n), seed=31)).cuda())
+    rng = np.random.default_rng(77)
+    gU = torch.from_numpy(rng.standard_normal((m, n))).cuda()
+    gs = torch.from_numpy(rng.standard_normal(n)).cuda()
+    gV = torch.from_numpy(rng.standard_normal((n, n))).cuda()
+
+    def step():
+        s, U, V = ddcsvd.svd_torch(A)
+        t1 = time.perf_counter()
+        ddcsvd.svd_backward_torch(U, s, V, gU, gs, gV)
+        torch.cuda.synchronize()
+        return t1
+
+    step()
+    tf, tb = [], []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t1 = step()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    tot = min(a + b for a, b in zip(tf, tb))
+    return {"config": "2048x1024 FP64, A = Q diag(linspace(1,2,1024)) P^T, cotangents N(0,1)",
+            "fwd_bwd_per_s": 1.0 / tot, "forward_ms": 1e3 * min(tf),
+            "backward_ms": 1e3 * min(tb)}
+
+
+def pmc_pass(m: int, n: int, seed: int) -> int:
+    """Workload for one rocprofv3 --pmc pass: three 1 GiB device copies (the
+    calibration kernel of known traffic), then one svd_gpu_dev call."""
+    import numpy as np
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+
+    ddcsvd.load()
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    mn = min(m, n)
+    A = ddcsvd.col_major(torch.from_numpy(gen.normal_matrix(m, n, seed)).cuda())
+    sigma = torch.empty(mn, dtype=torch.float64, device="cuda")
+    U = ddcsvd.col_major(torch.empty((m, mn), dtype=torch.float64, device="cuda"))
+    V = ddcsvd.col_major(torch.empty((n, mn), dtype=torch.float64, device="cuda"))
+    ddcsvd.svd_dev(A, sigma, U, V, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    print("pmc pass done", flush=True)
+    return 0
+
+
+def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
+    """Reads the counter CSVs of the two passes and writes
+    profiles/bd_left_gemv_{m}x{n}_traffic.json: HBM bytes per launch of
+    bd_left_gemv = (FETCH_SIZE + WRITE_SIZE) per dispatch, each scaled by
+    what the counter reads for the 1 GiB copy (known traffic 1 GiB each way)."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True)
+        per = {}
+        names = {}
+        for f in files:
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") != counter:
+                    continue
+                did = (f, r["Dispatch_Id"])
+                per[did] = per.get(did, 0.0) + float(r["Counter_Value"])
+                names[did] = r["Kernel_Name"]
+        return per, names
+
+    out = {"workload": f"one svd_gpu_dev call, {m}x{n} N(0,1) seed 2"}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        per, names = load(d, counter)
+        copy = [v for k, v in per.items() if "opy" in names[k] or "elementwise" in names[k]]
+        copy = sorted(copy)[-3:]   # the three 1 GiB copies are the largest copy dispatches
+        cal = (1 << 30) / (sum(copy) / len(copy)) if copy else None   # bytes per counter unit
+        left = [v for k, v in per.items() if "bd_left_gemv" in names[k]]
+        out[counter] = {"copy_counter_per_dispatch": sum(copy) / len(copy) if copy else None,
+                        "bytes_per_unit": cal,
+                        "left_gemv_dispatches": len(left),
+                        "left_gemv_counter_mean": sum(left) / len(left) if left else None}
+        out[counter]["left_gemv_bytes_per_launch"] = (
+            out[counter]["left_gemv_counter_mean"] * cal if left and cal else None)
+    f_b = out["FETCH_SIZE"]["left_gemv_bytes_per_launch"]
+    w_b = out["WRITE_SIZE"]["left_gemv_bytes_per_launch"]
+    out["hbm_bytes_per_launch"] = (f_b or 0.0) + (w_b or 0.0) if f_b else None
+    out["algorithmic_bytes_per_launch"] = left_gemv_bytes(m, n) / (n - 1)
+    path = os.path.join(ROOT, "profiles", f"bd_left_gemv_{m}x{n}_traffic.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="workload of one rocprofv3 --pmc pass (copy calibration + one SVD)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the traffic summary from two --pmc pass outputs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -96,7 +319,12 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-forms", action="store_true")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        return summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.m, args.n)
+    if args.pmc_pass:
+        return pmc_pass(args.m, args.n, args.seed)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(this build has no multi-GPU path)", file=sys.stderr)
@@ -108,10 +336,12 @@ def main() -> int:
     import ddcsvd
     from ddcsvd import gen
 
+    progress("imported torch")
     ddcsvd.load()
     m, n = args.m, args.n
     mn = min(m, n)
     A_host = gen.normal_matrix(m, n, args.seed)
+    progress(f"generated the {m}x{n} matrix")
     dev = torch.device("cuda:0")
     A = ddcsvd.col_major(torch.from_numpy(np.ascontiguousarray(A_host)).to(dev))
     sigma = torch.empty(mn, dtype=torch.float64, device=dev)
@@ -122,8 +352,8 @@ def main() -> int:
     for _ in range(args.warmup):
         ddcsvd.svd_dev(A, sigma, U, V, stream)
     torch.cuda.synchronize()
-    stage = {k: 0.0 for k in ("copy_in", "bidiag", "values", "vectors", "backtransform",
-                              "copy_out")}
+    progress("warmup done")
+    stage = {k: 0.0 for k in ddcsvd.TIMING_KEYS}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ddcsvd.svd_dev(A, sigma, U, V, stream)
@@ -132,9 +362,29 @@ def main() -> int:
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
+    progress(f"timed {args.steps} steps: {ms_per_step:.1f} ms per SVD")
+    # the same steps again with every GEMV launch bracketed by HIP events (two
+    # event records per launch cost about 10% of the SVD, so these steps are
+    # kept out of `value`)
+    ddcsvd.profile_kernels(True)
+    kt = {k: 0.0 for k in ddcsvd.TIMING_KEYS}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ddcsvd.svd_dev(A, sigma, U, V, stream)
+        for k, v in ddcsvd.last_timings().items():
+            kt[k] += v / args.steps
+    torch.cuda.synchronize()
+    ms_profiled = 1000.0 * (time.perf_counter() - t0) / args.steps
+    ddcsvd.profile_kernels(False)
+    progress(f"kernel-timed {args.steps} steps: {ms_profiled:.1f} ms per SVD")
 
+    mt, nt = max(m, n), mn
+    bl, br = left_gemv_bytes(mt, nt), right_gemv_bytes(mt, nt)
+    tl, tr = kt["left_gemv_ms"] / 1e3, kt["right_gemv_ms"] / 1e3
+    nl = max(kt["left_gemv_launches"], 1.0)
+    achieved = bl / tl / 1e9 if tl > 0 else None
+    traffic = traffic_summary(m, n)
     t_bd = stage["bidiag"] / 1000.0
-    achieved = 16.0 * S(m, n) / t_bd / 1e9
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -152,11 +402,23 @@ def main() -> int:
                                f"device-resident input (svd_gpu_dev)",
                    "m": m, "n": n, "seed": args.seed},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "kernel": "bidiagonalization stage (left/right GEMV + finalize kernels "
-                               "and the MFMA trailing update), 16*S(m,n) algorithmic bytes"},
-        "stages_ms": stage,
+                     "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": traffic,
+                     "kernel": "bd_left_gemv",
+                     "algorithmic_bytes_per_launch": bl / nl,
+                     "avg_launch_us": 1e6 * tl / nl if tl > 0 else None,
+                     "launches_per_svd": nl,
+                     "timing": f"HIP events around every launch over {args.steps} extra "
+                               f"steps ({ms_profiled:.0f} ms per SVD with the events)"},
+        "kernels": {
+            "bd_right_gemv": {"achieved_GBs": br / tr / 1e9 if tr > 0 else None,
+                              "avg_launch_us": 1e6 * tr / max(kt["right_gemv_launches"], 1),
+                              "algorithmic_bytes_per_svd": br},
+            "bidiag_stage": {"achieved_GBs": 16.0 * S(m, n) / t_bd / 1e9,
+                             "algorithmic_bytes_per_svd": 16 * S(m, n),
+                             "note": "two FP64 reads of the trailing matrix per step"}},
+        "stages_ms": {k: stage[k] for k in ddcsvd.TIMING_KEYS[:6]},
     }
     if not args.no_check:
         with torch.no_grad():
@@ -167,8 +429,16 @@ def main() -> int:
             out["orth_u"] = float(torch.linalg.norm(U.T @ U - eye))
             out["orth_v"] = float(torch.linalg.norm(V.T @ V - eye))
             out["sigma_max"] = float(s[-1])
+    del A, U, V, sigma
+    torch.cuda.empty_cache()
+    progress("checks done")
+    if not args.no_forms:
+        out["forms"] = {"batched": batched_form()}
+        out["forms"]["backward"] = backward_form()
+        progress("backward form done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, n, args.seed, args.cpu_steps)
+        progress("cpu baseline done")
     print(json.dumps(out), flush=True)
     return 0
 

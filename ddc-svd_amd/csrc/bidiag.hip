@@ -541,8 +541,10 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
     const int nchL = (int)ceil_div(m - rb0, RC);
     const int ncb = (int)ceil_div(n - k - 1, CBL);
     const int ndl = (int)ceil_div(2 * a.t, DPW);
+    const int kl0 = kclock_on() ? kclock_mark(st) : -1;
     hipLaunchKernelGGL(bd_left_gemv, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
     DDC_LAUNCHED();
+    if (kl0 >= 0) kclock_span(KCLOCK_LEFT_GEMV, kl0, kclock_mark(st));
     const int nfl = (int)ceil_div(n - k - 1, FIN);
     hipLaunchKernelGGL(bd_left_fin, dim3(nfl), dim3(FIN), 0, st, a, nchL);
     DDC_LAUNCHED();
@@ -557,8 +559,10 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
       const int nrb = (int)ceil_div(m - rbr, RBR);
       const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
+      const int kr0 = kclock_on() ? kclock_mark(st) : -1;
       hipLaunchKernelGGL(bd_right_gemv, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
       DDC_LAUNCHED();
+      if (kr0 >= 0) kclock_span(KCLOCK_RIGHT_GEMV, kr0, kclock_mark(st));
       hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();
     }

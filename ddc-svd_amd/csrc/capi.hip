@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "../../include/svd_gpu.h"
 #include "ddc_common.h"
@@ -18,7 +20,18 @@ namespace ddc {
 namespace {
 std::mutex g_mu;
 char g_err[512] = "";
-double g_times[6] = {0, 0, 0, 0, 0, 0};
+constexpr int kNumTimes = 10;
+double g_times[kNumTimes] = {};
+
+// Per-launch kernel clock (see kclock_on in ddc_common.h).
+struct KClock {
+  bool enabled = false;   // requested by svd_gpu_profile_kernels
+  bool active = false;    // inside a timed call
+  std::vector<hipEvent_t> pool;
+  int used = 0;
+  bool failed = false;
+  std::vector<std::pair<int, int>> spans[KCLOCK_CLASSES];
+} g_kc;
 struct Ctx {
   bool init = false;
   int dev = 0;
@@ -34,6 +47,56 @@ void set_err(const char* fmt, ...) {
   va_end(ap);
 }
 }  // namespace
+
+bool kclock_on() { return g_kc.active && !g_kc.failed; }
+
+int kclock_mark(hipStream_t st) {
+  if (g_kc.used == (int)g_kc.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      g_kc.failed = true;
+      return -1;
+    }
+    g_kc.pool.push_back(e);
+  }
+  if (hipEventRecord(g_kc.pool[g_kc.used], st) != hipSuccess) {
+    g_kc.failed = true;
+    return -1;
+  }
+  return g_kc.used++;
+}
+
+void kclock_span(int cls, int a, int b) {
+  if (a >= 0 && b >= 0 && cls >= 0 && cls < KCLOCK_CLASSES) g_kc.spans[cls].emplace_back(a, b);
+}
+
+static void kclock_begin() {
+  g_kc.active = g_kc.enabled;
+  g_kc.failed = false;
+  g_kc.used = 0;
+  for (auto& v : g_kc.spans) v.clear();
+}
+
+// The clock runs only inside one svd call, whatever path leaves it.
+struct KClockScope {
+  KClockScope() { kclock_begin(); }
+  ~KClockScope() { g_kc.active = false; }
+};
+
+// After the stream has drained: milliseconds and launch counts per class
+// into g_times[6..9].
+static void kclock_collect() {
+  for (int c = 0; c < KCLOCK_CLASSES; ++c) {
+    double ms = 0.0;
+    for (auto& pr : g_kc.spans[c]) {
+      float t = 0.f;
+      if (hipEventElapsedTime(&t, g_kc.pool[pr.first], g_kc.pool[pr.second]) == hipSuccess) ms += t;
+    }
+    g_times[6 + c] = g_kc.active && !g_kc.failed ? ms : 0.0;
+    g_times[8 + c] = g_kc.active && !g_kc.failed ? (double)g_kc.spans[c].size() : 0.0;
+  }
+  g_kc.active = false;
+}
 
 void report_error(const char* msg) { set_err("%s", msg); }
 void clear_error() { g_err[0] = 0; }
@@ -143,7 +206,8 @@ struct Timer {
     return DDC_OK;
   }
   void finish() {
-    for (int i = 0; i < 6; ++i) g_times[i] = 0.0;
+    for (int i = 0; i < kNumTimes; ++i) g_times[i] = 0.0;
+    kclock_collect();
     for (int i = 0; i + 1 < n && i < 6; ++i) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) g_times[i] = ms;
@@ -213,6 +277,7 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
   double* Araw = wide ? ws.take<double>((size_t)m * n) : nullptr;
   Timer tm;
   DDC_TRY(tm.create(st));
+  KClockScope kcs;
   DDC_TRY(tm.mark());
   DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * nt, st));
   if (!wide) {
@@ -262,6 +327,7 @@ static int svd_dev(int m, int n, const double* dA, int lda, double* dsigma, doub
   DDC_TRY(tall_take(ws, mt, nt, wide, b));
   Timer tm;
   DDC_TRY(tm.create(st));
+  KClockScope kcs;
   DDC_TRY(tm.mark());
   DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * nt, st));
   if (!wide) {
@@ -470,9 +536,15 @@ int svd_gpu_backward(int m, int n, const double* U, int ldu, const double* sigma
 
 int svd_gpu_last_timings(double* ms, int count) {
   std::lock_guard<std::mutex> lk(g_mu);
-  int k = count < 6 ? count : 6;
+  int k = count < kNumTimes ? count : kNumTimes;
   for (int i = 0; i < k; ++i) ms[i] = g_times[i];
   return k;
+}
+
+int svd_gpu_profile_kernels(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_kc.enabled = on != 0;
+  return DDC_OK;
 }
 
 const char* svd_gpu_last_error(void) { return g_err; }

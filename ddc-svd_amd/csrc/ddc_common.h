@@ -24,6 +24,15 @@ namespace ddc {
 // Records the first HIP failure of the calling thread so entry points can
 // report it; returns DDC_E_HIP on failure.
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+// Optional HIP-event timing of single kernel launches (for measurement; off
+// unless svd_gpu_profile_kernels(1) was called).  kclock_on() tells whether
+// it is active; kclock_mark() records an event on st and returns its index;
+// kclock_span(cls, a, b) books the span between two marks to class cls.
+bool kclock_on();
+int kclock_mark(hipStream_t st);
+void kclock_span(int cls, int a, int b);
+enum { KCLOCK_LEFT_GEMV = 0, KCLOCK_RIGHT_GEMV = 1, KCLOCK_CLASSES = 2 };
+
 // Message returned by svd_gpu_last_error(); clear_error() empties it.
 void report_error(const char* msg);
 void clear_error();

@@ -73,6 +73,8 @@ def load():
         lib.svd_gpu_backward.restype = i
         lib.svd_gpu_last_timings.argtypes = [d, i]
         lib.svd_gpu_last_timings.restype = i
+        lib.svd_gpu_profile_kernels.argtypes = [i]
+        lib.svd_gpu_profile_kernels.restype = i
         lib.svd_gpu_last_error.argtypes = []
         lib.svd_gpu_last_error.restype = ctypes.c_char_p
         _lib = lib
@@ -92,12 +94,21 @@ def last_error() -> str:
     return load().svd_gpu_last_error().decode()
 
 
+TIMING_KEYS = ["copy_in", "bidiag", "values", "vectors", "backtransform", "copy_out",
+               "left_gemv_ms", "right_gemv_ms", "left_gemv_launches", "right_gemv_launches"]
+
+
 def last_timings() -> dict:
-    """Stage times (ms) of the last svd call."""
-    out = np.zeros(6)
-    load().svd_gpu_last_timings(_p(out), 6)
-    return dict(zip(["copy_in", "bidiag", "values", "vectors", "backtransform", "copy_out"],
-                    out.tolist()))
+    """Stage times (ms) of the last svd call (and, with profile_kernels(True),
+    the summed GEMV launch times and counts)."""
+    out = np.zeros(len(TIMING_KEYS))
+    load().svd_gpu_last_timings(_p(out), len(TIMING_KEYS))
+    return dict(zip(TIMING_KEYS, out.tolist()))
+
+
+def profile_kernels(on: bool) -> None:
+    """Bracket every bidiagonalization GEMV launch with HIP events."""
+    load().svd_gpu_profile_kernels(1 if on else 0)
 
 
 def svd(A: np.ndarray, raw: bool = False):

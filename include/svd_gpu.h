@@ -158,11 +158,19 @@ int svd_gpu_batched_set_config(svd_gpu_batch *plan, int variant, int wgs_per_cu,
                                int coop_min_n);
 void svd_gpu_batched_destroy(svd_gpu_batch *plan);
 
-/* Milliseconds of the stages of the last successful call on this thread's
- * process: [0] copy-in, [1] bidiagonalization, [2] singular values,
- * [3] vectors, [4] back-transformation, [5] copy-out.  Returns the number of
- * entries written (at most count). */
+/* Milliseconds of the stages of the last successful svd_gpu / svd_gpu_status
+ * / svd_gpu_dev call in this process: [0] copy-in, [1] bidiagonalization,
+ * [2] singular values, [3] vectors, [4] back-transformation, [5] copy-out.
+ * With kernel profiling on (svd_gpu_profile_kernels): [6] / [7] summed
+ * durations of the left / right GEMV launches of the bidiagonalization and
+ * [8] / [9] their launch counts (zeros otherwise).  Returns the number of
+ * entries written (at most count, at most 10). */
 int svd_gpu_last_timings(double *ms, int count);
+
+/* Measurement hook (no reference counterpart): on != 0 brackets every GEMV
+ * launch of the bidiagonalization with HIP events on its stream, reported by
+ * svd_gpu_last_timings; costs two event records per launch. */
+int svd_gpu_profile_kernels(int on);
 
 /* Human-readable description of the last failure (empty if none). */
 const char *svd_gpu_last_error(void);
