@@ -23,7 +23,9 @@ HBM bytes per launch from the committed rocprofv3 counter summary
 first steps of the same matrix and scales by the algorithmic work ratio.
 `forms` holds the batched SVD (10,000 matrices, m ~ U{16..256}, n ~ U{8..m},
 entries U(-1,1)) in FP64 and FP32 against a loop of plain svd_gpu_dev calls,
-and the forward + backward throughput of the differentiable SVD at 2048 x 1024.
+the forward + backward throughput of the differentiable SVD at 2048 x 1024,
+and the mixed-precision SVD of an 8192^2 matrix with singular values graded
+from 1 to 1e-6 against the plain FP64 path (errors per decade of sigma).
 """
 from __future__ import annotations
 
@@ -304,6 +306,71 @@ def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
     return 0
 
 
+def mixed_form(n: int = 8192, seed: int = 3) -> dict:
+    """A = Q diag(sigma) P^T with Haar Q, P (QR of seeded N(0,1), columns
+    signed by diag R) and sigma_i = 10^(-6 i/(n-1)); plain FP64 against the
+    mixed forms, errors per singular value against the exact sigma."""
+    import numpy as np
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+
+    sig = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+    exact = np.sort(sig)
+    qs = []
+    for k in range(2):
+        g = torch.from_numpy(gen.normal_matrix(n, n, seed + k)).cuda()
+        q, r = torch.linalg.qr(g)
+        qs.append(q * torch.sign(torch.diagonal(r)))
+        del g, r
+    A = ddcsvd.col_major((qs[0] * torch.from_numpy(sig).cuda()) @ qs[1].T)
+    del qs
+    torch.cuda.synchronize()
+    progress("mixed: generated the graded matrix")
+    bands = [(10.0 ** -(d + 1), 10.0 ** -d) for d in range(6)]
+
+    def report(s, U, V, secs):
+        sh = s.cpu().numpy()
+        err = np.abs(sh - exact)
+        res = float(torch.linalg.norm(A - (U * s) @ V.T) / torch.linalg.norm(A))
+        eye = torch.eye(n, dtype=torch.float64, device="cuda")
+        orth = float(max(torch.linalg.norm(U.T @ U - eye), torch.linalg.norm(V.T @ V - eye)))
+        per_band = {}
+        for lo, hi in bands:
+            sel = (exact >= lo * (1 - 1e-12)) & (exact <= hi * (1 + 1e-12))
+            per_band[f"[{lo:.0e},{hi:.0e}]"] = {
+                "max_abs_rel_smax": float(err[sel].max()),
+                "max_rel": float(np.max(err[sel] / exact[sel])),
+                "median_rel": float(np.median(err[sel] / exact[sel]))}
+        return {"ms": 1e3 * secs, "svd_per_s": 1.0 / secs, "residual": res, "orthogonality": orth,
+                "max_abs_rel_smax": float(err.max()), "max_rel": float(np.max(err / exact)),
+                "per_decade": per_band}
+
+    out = {"config": f"{n}x{n} FP64, A = Q diag(10^(-6 i/{n - 1})) P^T, Haar Q, P from the QR "
+                     f"of N(0,1) seeds {seed}, {seed + 1}; errors against the exact sigma"}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    (s, U, V), t = timed(lambda: ddcsvd.svd_torch(A))
+    out["plain_fp64"] = report(s, U, V, t)
+    progress("mixed: plain FP64 done")
+    for name, upd, steps in (("f32_1step", "f32", 1), ("f32_2steps", "f32", 2),
+                             ("bf16x3_1step", "bf16x3", 1)):
+        (s, U, V, dg), t = timed(lambda: ddcsvd.svd_mixed_torch(A, upd, steps))
+        out[name] = report(s, U, V, t)
+        out[name]["stages"] = dg
+        progress(f"mixed: {name} done ({1e3 * t:.0f} ms)")
+    out["speedup_f32_1step_vs_plain"] = out["plain_fp64"]["ms"] / out["f32_1step"]["ms"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--pmc-pass", action="store_true",
@@ -431,11 +498,23 @@ def main() -> int:
             out["sigma_max"] = float(s[-1])
     del A, U, V, sigma
     torch.cuda.empty_cache()
+    # the drop-in host-buffer entry point (svd_gpu_status): PCIe copies included
+    t0 = time.perf_counter()
+    ddcsvd.svd(A_host)
+    host_ms = 1e3 * (time.perf_counter() - t0)
+    ht = ddcsvd.last_timings()
+    out["host_buffers"] = {"ms_per_svd": host_ms, "svd_per_s": 1e3 / host_ms,
+                           "copy_in_ms": ht["copy_in"], "copy_out_ms": ht["copy_out"],
+                           "note": "svd_gpu_status with host A, sigma, U, V (one call; "
+                                   "includes the host-side U and V allocations of the "
+                                   "Python wrapper)"}
+    progress(f"host-buffer entry point: {host_ms:.1f} ms")
     progress("checks done")
     if not args.no_forms:
         out["forms"] = {"batched": batched_form()}
         out["forms"]["backward"] = backward_form()
         progress("backward form done")
+        out["forms"]["mixed"] = mixed_form()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, n, args.seed, args.cpu_steps)
         progress("cpu baseline done")

@@ -34,7 +34,7 @@ FLAGS = [
 ]
 
 SOURCES = ["blas.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
-           "adjoint.hip", "batched.hip", "capi.hip"]
+           "adjoint.hip", "batched.hip", "mixed.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

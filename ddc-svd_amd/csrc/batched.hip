@@ -676,7 +676,7 @@ static const BjVariant kVariants[] = {
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 static const int kWgsPerCu[] = {2, 3};
 // first entry: the threshold used while the variants are compared
-static const int kCoopNt[] = {32, 16, 64, 128, BJ_MAXDIM + 1};
+static const int kCoopNt[] = {32, 1, 8, 16, 64, BJ_MAXDIM + 1};
 
 // A launch configuration: kernel variant, resident workgroups per CU, and
 // the smallest min(m, n) a whole workgroup factors together.

@@ -38,10 +38,13 @@ constexpr int DPW = 8;        // panel dot products per dot work-group
 constexpr int FIN = 64;       // items per finalize work-group (one wave)
 constexpr int DSTRIDE = 2 * NB + 1;
 
+// TS: storage type of the matrix and of the reflectors kept in it (double,
+// or float for the mixed-precision form); all vector arithmetic is FP64.
+template <typename TS>
 struct StepArgs {
   int m, n, lda, ldy;
   int k, p, t;
-  double* A;
+  TS* A;
   double* alpha;
   double* beta;
   double* Y;  // n x NB, ld ldy: y_s
@@ -57,6 +60,15 @@ struct StepArgs {
   double* rnorm; // partial |r|^2 [nrn]
   int ncn, nrn;
 };
+
+// Two consecutive stored entries as doubles (p 2-element aligned).
+__device__ __forceinline__ double2 ld2(const double* p) {
+  return *reinterpret_cast<const double2*>(p);
+}
+__device__ __forceinline__ double2 ld2(const float* p) {
+  const float2 v = *reinterpret_cast<const float2*>(p);
+  return make_double2((double)v.x, (double)v.y);
+}
 
 __device__ __forceinline__ double block_sum256(double v, double* red) {
   v = wave_sum(v);
@@ -115,18 +127,20 @@ struct Refl {
 
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void bd_init(int m, const double* A,
+template <typename TS>
+__global__ __launch_bounds__(256) void bd_init(int m, const TS* A,
                                                double* cvec, double* cnorm) {
   __shared__ double red[4];
   int row = blockIdx.x * 256 + threadIdx.x;
-  double c = (row < m) ? A[row] : 0.0;
+  double c = (row < m) ? (double)A[row] : 0.0;
   if (row < m) cvec[row] = c;
   double s = block_sum256(c * c, red);
   if (threadIdx.x == 0) cnorm[blockIdx.x] = s;
 }
 
 // left_gemv: grid.x = ncb column blocks + dot blocks, grid.y = row chunks.
-__global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0) {
+template <typename TS>
+__global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int rb0) {
   __shared__ double red[4][CBL];
   __shared__ double s_scal;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -167,12 +181,12 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
     for (int c = 0; c < CBL; ++c) {
       acc[c] = 0.0;
       if (c < jn) {
-        const double* col = a.A + (int64_t)(j0 + c) * a.lda;
+        const TS* col = a.A + (int64_t)(j0 + c) * a.lda;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int r = r0 + 512 * q;
           if (r < a.lda) {
-            double2 v = *reinterpret_cast<const double2*>(col + r);
+            const double2 v = ld2(col + r);
             acc[c] += v.x * u[2 * q] + v.y * u[2 * q + 1];
           }
         }
@@ -194,7 +208,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = r0 + 512 * q + h;
-          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = u[2 * q + h];
+          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = (TS)u[2 * q + h];
         }
       if (blockIdx.y == 0 && tid == 0) a.alpha[k] = R.diag();
     }
@@ -208,14 +222,25 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
       acc[e] = 0.0;
       const int s = q0 + e;
       if (s < nd) {
-        const double* col = (s < a.t) ? a.A + (int64_t)(a.p + s) * a.lda
-                                      : a.X + (int64_t)(s - a.t) * a.lda;
+        if (s < a.t) {
+          const TS* col = a.A + (int64_t)(a.p + s) * a.lda;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = r0 + 512 * q;
-          if (r < a.lda) {
-            double2 v = *reinterpret_cast<const double2*>(col + r);
-            acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + 512 * q;
+            if (r < a.lda) {
+              const double2 v = ld2(col + r);
+              acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
+            }
+          }
+        } else {
+          const double* col = a.X + (int64_t)(s - a.t) * a.lda;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + 512 * q;
+            if (r < a.lda) {
+              const double2 v = ld2(col + r);
+              acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
+            }
           }
         }
       }
@@ -233,7 +258,8 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs a, int ncb, int rb0
 }
 
 // left_fin: one thread per trailing column j in [k+1, n), one wave per group.
-__global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs a, int nchL) {
+template <typename TS>
+__global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
   __shared__ double dd[2 * NB];  // u_s^T u, x_s^T u
   __shared__ double uk[NB + 1];  // u_s[k], s <= t
   __shared__ double xk[NB];      // x_s[k], s < t
@@ -243,12 +269,12 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs a, int nchL) {
   R.make(sum_partials(a.cnorm, a.ncn), a.cvec[k]);
   if (tid < 2 * t) {
     double s = sum_strided(a.Ldot + tid, nchL, 2 * NB);
-    double w1 = (tid < t) ? a.A[k + (int64_t)(a.p + tid) * a.lda]
+    double w1 = (tid < t) ? (double)a.A[k + (int64_t)(a.p + tid) * a.lda]
                           : a.X[k + (int64_t)(tid - t) * a.lda];
     dd[tid] = R.dot(s, w1);
   }
   if (tid < t) {
-    uk[tid] = a.A[k + (int64_t)(a.p + tid) * a.lda];
+    uk[tid] = (double)a.A[k + (int64_t)(a.p + tid) * a.lda];
     xk[tid] = a.X[k + (int64_t)tid * a.lda];
   }
   if (tid == t) uk[t] = R.vec(a.cvec[k], true);
@@ -258,7 +284,7 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs a, int nchL) {
   double rr = 0.0;
   if (j < a.n) {
     double y = sum_strided(a.Lpart + j, nchL, a.ldy);
-    double r = a.A[k + (int64_t)j * a.lda];
+    double r = (double)a.A[k + (int64_t)j * a.lda];
     const double* Yj = a.Y + j;
     const double* Vj = a.V + j;
     for (int s0 = 0; s0 < t; s0 += 8) {
@@ -289,7 +315,8 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs a, int nchL) {
 }
 
 // right_gemv: grid.x = row blocks + dot blocks, grid.y = column chunks.
-__global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb0) {
+template <typename TS>
+__global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0) {
   __shared__ double vs[CCR];
   __shared__ double red[4][DPW];
   __shared__ double s_scal;
@@ -310,14 +337,13 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb
   if ((int)blockIdx.x < nrb) {
     const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
     if (r0 < a.lda) {
-      const double* base = a.A + r0 + (int64_t)j0 * a.lda;
+      const TS* base = a.A + r0 + (int64_t)j0 * a.lda;
       double h0 = 0.0, h1 = 0.0;
       int q = 0;
       for (; q + 8 <= jn; q += 8) {
         double2 v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          v[u] = *reinterpret_cast<const double2*>(base + (int64_t)(q + u) * a.lda);
+        for (int u = 0; u < 8; ++u) v[u] = ld2(base + (int64_t)(q + u) * a.lda);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           h0 += v[u].x * vs[q + u];
@@ -325,7 +351,7 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb
         }
       }
       for (; q < jn; ++q) {
-        double2 v = *reinterpret_cast<const double2*>(base + (int64_t)q * a.lda);
+        const double2 v = ld2(base + (int64_t)q * a.lda);
         h0 += v.x * vs[q];
         h1 += v.y * vs[q];
       }
@@ -336,7 +362,7 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb
     if (blockIdx.x == 0) {
       __syncthreads();  // row k of this chunk has been read by the loop above
       if (tid < jn) {
-        a.A[k + (int64_t)(j0 + tid) * a.lda] = vs[tid];
+        a.A[k + (int64_t)(j0 + tid) * a.lda] = (TS)vs[tid];
         a.V[(j0 + tid) + (int64_t)a.t * a.ldy] = vs[tid];
       }
       if (blockIdx.y == 0 && tid == 0) a.beta[k] = R.diag();
@@ -373,7 +399,8 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs a, int nrb, int rb
 // right_fin: one thread per row in [k+1, m), one wave per group.  noright:
 // the row of M has a single entry, taken as beta with no reflection
 // (bidiag.c:125-129).
-__global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs a, int nccR, int noright) {
+template <typename TS>
+__global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright) {
   __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
   __shared__ double yk1[NB];       // Y[k+1, s], s <= t
   __shared__ double vk1[NB];       // V[k+1, s], s <= t
@@ -400,16 +427,16 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs a, int nccR, int no
   double cc2 = 0.0;
   if (row < a.m) {
     double x = 0.0;
-    double c = next ? a.A[row + (int64_t)(k + 1) * a.lda] : 0.0;
+    double c = next ? (double)a.A[row + (int64_t)(k + 1) * a.lda] : 0.0;
     if (!noright) x = sum_strided(a.Rpart + row, nccR, a.lda);
-    const double* Ur = a.A + row + (int64_t)a.p * a.lda;
+    const TS* Ur = a.A + row + (int64_t)a.p * a.lda;
     const double* Xr = a.X + row;
     for (int s0 = 0; s0 < t; s0 += 8) {
       double us[8], xs[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int s = min(s0 + u, t - 1);  // clamped: in bounds, unused past t
-        us[u] = Ur[(int64_t)s * a.lda];
+        us[u] = (double)Ur[(int64_t)s * a.lda];
         xs[u] = Xr[(int64_t)s * a.lda];
       }
 #pragma unroll
@@ -421,7 +448,7 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs a, int nccR, int no
         }
       }
     }
-    double ut = Ur[(int64_t)t * a.lda];
+    double ut = (double)Ur[(int64_t)t * a.lda];
     if (!noright) {
       x -= ut * dyv[t];
       x *= 2.0;
@@ -437,14 +464,15 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs a, int nccR, int no
   if (tid == 0) a.cnorm[blockIdx.x] = s;
   if (noright && blockIdx.x == 0 && tid == 0) {
     a.beta[k] = a.rvec[k + 1];
-    a.A[k + (int64_t)(k + 1) * a.lda] = 0.0;
+    a.A[k + (int64_t)(k + 1) * a.lda] = (TS)0;
     a.V[(k + 1) + (int64_t)t * a.ldy] = 0.0;
   }
 }
 
 // Last column (m >= n): left reflector only, the 1 x 1 case included
 // (bidiag_par.c:384-397).
-__global__ __launch_bounds__(256) void bd_final_left(StepArgs a) {
+template <typename TS>
+__global__ __launch_bounds__(256) void bd_final_left(StepArgs<TS> a) {
   __shared__ double s_scal;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k;
@@ -456,7 +484,7 @@ __global__ __launch_bounds__(256) void bd_final_left(StepArgs a) {
   Refl R;
   R.make(s_scal, a.cvec[k]);
   const int row = k + blockIdx.x * 256 + tid;
-  if (row < a.m) a.A[row + (int64_t)k * a.lda] = R.vec(a.cvec[row], row == k);
+  if (row < a.m) a.A[row + (int64_t)k * a.lda] = (TS)R.vec(a.cvec[row], row == k);
   if (blockIdx.x == 0 && tid == 0) a.alpha[k] = R.diag();
 }
 
@@ -476,7 +504,59 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
   return d;
 }
 
-void bidiag_plan(ArenaPlan& p, int m, int n, int lda) {
+// ---- operands of the reduced-precision trailing update ----------------------
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// Entry (i, c) of the update operand, i < rows, c < 2 NB:
+//   side 0: [U X] (rows r0.. of the panel's reflectors, stored as float, and X)
+//   side 1: [Y V] (contiguous double columns)
+__device__ __forceinline__ float upd_entry(int side, int i, int c, const float* U, int ldu,
+                                           const double* D, int ldd) {
+  if (side == 0 && c < NB) return U[i + (int64_t)c * ldu];
+  const int cc = side == 0 ? c - NB : c;
+  return (float)D[i + (int64_t)cc * ldd];
+}
+
+// FP32 copy of an update operand (rows x 2NB, ld ldo).
+__global__ void upd_pack_f32(int side, int rows, const float* U, int ldu, const double* D,
+                             int ldd, float* out, int ldo) {
+  const int64_t total = (int64_t)rows * 2 * NB;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / rows), i = (int)(e - (int64_t)c * rows);
+    out[i + (int64_t)c * ldo] = upd_entry(side, i, c, U, ldu, D, ldd);
+  }
+}
+
+// BF16 operand.  pieces = 1: hi parts (K = 2NB).  pieces = 3 (split BF16):
+// K = 6NB laid out so that [hi hi lo] (side 0) times [hi lo hi]^T (side 1)
+// sums hi*hi + hi*lo + lo*hi, x = hi + lo the two-term BF16 split.
+__global__ void upd_pack_bf16(int side, int pieces, int rows, const float* U, int ldu,
+                              const double* D, int ldd, unsigned short* out, int ldo) {
+  const int64_t total = (int64_t)rows * 2 * NB;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / rows), i = (int)(e - (int64_t)c * rows);
+    const float x = upd_entry(side, i, c, U, ldu, D, ldd);
+    const unsigned short hi = bf16_rne(x);
+    out[i + (int64_t)c * ldo] = hi;
+    if (pieces == 3) {
+      const unsigned short lo = bf16_rne(x - bf16_val(hi));
+      out[i + (int64_t)(c + 2 * NB) * ldo] = side == 0 ? hi : lo;
+      out[i + (int64_t)(c + 4 * NB) * ldo] = side == 0 ? lo : hi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+static void bidiag_plan_common(ArenaPlan& p, int m, int n, int lda) {
   BidiagBufs d = bidiag_dims(m, n, lda);
   p.add<double>((size_t)d.ldy * NB * 2);   // Y, V
   p.add<double>((size_t)lda * NB * 2);     // copy of the panel's U, then X
@@ -490,11 +570,27 @@ void bidiag_plan(ArenaPlan& p, int m, int n, int lda) {
   p.add<double>(d.nrn_max);
 }
 
-int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
-               Arena& ws, hipStream_t st) {
+void bidiag_plan(ArenaPlan& p, int m, int n, int lda) { bidiag_plan_common(p, m, n, lda); }
+
+void bidiag_plan_f32(ArenaPlan& p, int m, int n, int lda, int upd) {
+  bidiag_plan_common(p, m, n, lda);
+  const int ldy = (int)round_up(n, 16);
+  if (upd == UPD_F32) {
+    p.add<float>((size_t)lda * 2 * NB);
+    p.add<float>((size_t)ldy * 2 * NB);
+  } else {
+    const int pieces = upd == UPD_BF16X3 ? 3 : 1;
+    p.add<unsigned short>((size_t)lda * 2 * NB * pieces);
+    p.add<unsigned short>((size_t)ldy * 2 * NB * pieces);
+  }
+}
+
+template <typename TS>
+static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* beta, int upd,
+                        Arena& ws, hipStream_t st) {
   if (m < n || n < 1 || lda < m || (lda % 2) != 0) return DDC_E_ARG;
   BidiagBufs d = bidiag_dims(m, n, lda);
-  StepArgs a;
+  StepArgs<TS> a;
   a.m = m;
   a.n = n;
   a.lda = lda;
@@ -517,6 +613,20 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
   a.cnorm = ws.take<double>(d.ncn_max);
   a.rnorm = ws.take<double>(d.nrn_max);
   if (!a.rnorm) return DDC_E_NOMEM;
+  float *opA32 = nullptr, *opB32 = nullptr;
+  unsigned short *opA16 = nullptr, *opB16 = nullptr;
+  const int pieces = upd == UPD_BF16X3 ? 3 : 1;
+  if constexpr (sizeof(TS) == 4) {
+    if (upd == UPD_F32) {
+      opA32 = ws.take<float>((size_t)lda * 2 * NB);
+      opB32 = ws.take<float>((size_t)d.ldy * 2 * NB);
+      if (!opB32) return DDC_E_NOMEM;
+    } else {
+      opA16 = ws.take<unsigned short>((size_t)lda * 2 * NB * pieces);
+      opB16 = ws.take<unsigned short>((size_t)d.ldy * 2 * NB * pieces);
+      if (!opB16) return DDC_E_NOMEM;
+    }
+  }
 
   DDC_HIP(hipMemsetAsync(a.Y, 0, sizeof(double) * (size_t)d.ldy * NB * 2, st));
   DDC_HIP(hipMemsetAsync(a.X, 0, sizeof(double) * (size_t)lda * NB, st));
@@ -524,7 +634,7 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
 
   a.ncn = (int)ceil_div(m, 256);
   a.nrn = 0;
-  hipLaunchKernelGGL(bd_init, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
+  hipLaunchKernelGGL(bd_init<TS>, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
   DDC_LAUNCHED();
 
   for (int k = 0; k < n; ++k) {
@@ -532,7 +642,8 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
     a.p = (k / NB) * NB;
     a.t = k - a.p;
     if (k == n - 1) {
-      hipLaunchKernelGGL(bd_final_left, dim3((unsigned)ceil_div(m - k, 256)), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(bd_final_left<TS>, dim3((unsigned)ceil_div(m - k, 256)), dim3(256), 0,
+                         st, a);
       DDC_LAUNCHED();
       break;
     }
@@ -542,17 +653,17 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
     const int ncb = (int)ceil_div(n - k - 1, CBL);
     const int ndl = (int)ceil_div(2 * a.t, DPW);
     const int kl0 = kclock_on() ? kclock_mark(st) : -1;
-    hipLaunchKernelGGL(bd_left_gemv, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
+    hipLaunchKernelGGL(bd_left_gemv<TS>, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
     DDC_LAUNCHED();
     if (kl0 >= 0) kclock_span(KCLOCK_LEFT_GEMV, kl0, kclock_mark(st));
     const int nfl = (int)ceil_div(n - k - 1, FIN);
-    hipLaunchKernelGGL(bd_left_fin, dim3(nfl), dim3(FIN), 0, st, a, nchL);
+    hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);
     DDC_LAUNCHED();
     a.nrn = nfl;
     // right reflector of row k and the right GEMV
     const int nfr = (int)ceil_div(m - k - 1, FIN);
     if (k == n - 2) {
-      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
+      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
       DDC_LAUNCHED();
     } else {
       const int rbr = ((k + 1) / 64) * 64;
@@ -560,24 +671,60 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
       const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
       const int kr0 = kclock_on() ? kclock_mark(st) : -1;
-      hipLaunchKernelGGL(bd_right_gemv, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
+      hipLaunchKernelGGL(bd_right_gemv<TS>, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
       DDC_LAUNCHED();
       if (kr0 >= 0) kclock_span(KCLOCK_RIGHT_GEMV, kr0, kclock_mark(st));
-      hipLaunchKernelGGL(bd_right_fin, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
+      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();
     }
     a.ncn = nfr;
     // blocked trailing update at the end of a full panel
     if (a.t == NB - 1 && a.p + NB < n) {
       const int r0 = a.p + NB;
-      DDC_HIP(hipMemcpy2DAsync(UX + r0, sizeof(double) * lda, A + r0 + (int64_t)a.p * lda,
-                               sizeof(double) * lda, sizeof(double) * (m - r0), NB,
-                               hipMemcpyDeviceToDevice, st));
-      DDC_TRY(dgemm('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0, d.ldy, 1.0,
-                    A + r0 + (int64_t)r0 * lda, lda, st));
+      TS* C = A + r0 + (int64_t)r0 * lda;
+      if constexpr (sizeof(TS) == 8) {
+        DDC_HIP(hipMemcpy2DAsync(UX + r0, sizeof(double) * lda, A + r0 + (int64_t)a.p * lda,
+                                 sizeof(double) * lda, sizeof(double) * (m - r0), NB,
+                                 hipMemcpyDeviceToDevice, st));
+        DDC_TRY(dgemm('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0, d.ldy,
+                      1.0, C, lda, st));
+      } else {
+        const float* Up = A + r0 + (int64_t)a.p * lda;
+        const int ra = m - r0, rb = n - r0;
+        const unsigned ga = (unsigned)std::min<int64_t>(ceil_div((int64_t)ra * 2 * NB, 256), 4096);
+        const unsigned gb = (unsigned)std::min<int64_t>(ceil_div((int64_t)rb * 2 * NB, 256), 4096);
+        if (upd == UPD_F32) {
+          hipLaunchKernelGGL(upd_pack_f32, dim3(ga), dim3(256), 0, st, 0, ra, Up, lda,
+                             a.X + r0, lda, opA32, lda);
+          hipLaunchKernelGGL(upd_pack_f32, dim3(gb), dim3(256), 0, st, 1, rb, Up, lda,
+                             a.Y + r0, d.ldy, opB32, d.ldy);
+          DDC_LAUNCHED();
+          DDC_TRY(sgemm('N', 'T', ra, rb, 2 * NB, -1.0f, opA32, lda, opB32, d.ldy, 1.0f, C, lda,
+                        st));
+        } else {
+          hipLaunchKernelGGL(upd_pack_bf16, dim3(ga), dim3(256), 0, st, 0, pieces, ra, Up, lda,
+                             a.X + r0, lda, opA16, lda);
+          hipLaunchKernelGGL(upd_pack_bf16, dim3(gb), dim3(256), 0, st, 1, pieces, rb, Up, lda,
+                             a.Y + r0, d.ldy, opB16, d.ldy);
+          DDC_LAUNCHED();
+          DDC_TRY(gemm_bf16_f32('N', 'T', ra, rb, 2 * NB * pieces, -1.0f, opA16, lda, opB16,
+                                d.ldy, 1.0f, C, lda, st));
+        }
+      }
     }
   }
   return DDC_OK;
+}
+
+int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta, Arena& ws,
+               hipStream_t st) {
+  return bidiag_drive<double>(m, n, A, lda, alpha, beta, 0, ws, st);
+}
+
+int bidiag_run_f32(int m, int n, float* A, int lda, double* alpha, double* beta, int upd,
+                   Arena& ws, hipStream_t st) {
+  if (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16) return DDC_E_ARG;
+  return bidiag_drive<float>(m, n, A, lda, alpha, beta, upd, ws, st);
 }
 
 }  // namespace ddc

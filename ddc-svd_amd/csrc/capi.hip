@@ -468,6 +468,37 @@ static int backward_dev(int m, int n, const double* U, int ldu, const double* si
   return DDC_OK;
 }
 
+static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
+                     double* dV, int ldv, int upd, int steps, double* diag, hipStream_t user) {
+  DDC_TRY(check_dims(n, n));
+  if (!dA || !dsigma || !dU || !dV || lda < n || ldu < n || ldv < n || steps < 0 || steps > 8 ||
+      (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16)) {
+    set_err("invalid svd_gpu_mixed arguments");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = user ? user : g_ctx.st;
+  ArenaPlan plan;
+  plan.add<int>(64);
+  mixed_plan(plan, n, upd);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  int* flag = ws.take<int>(64);
+  if (!flag) return DDC_E_NOMEM;
+  DDC_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, n, n, dA, lda, flag);
+  DDC_LAUNCHED();
+  int hflag = 0;
+  DDC_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  if (hflag) {
+    set_err("input matrix holds NaN or Inf");
+    return DDC_E_NUMERIC;
+  }
+  DDC_TRY(mixed_run(n, dA, lda, dsigma, dU, ldu, dV, ldv, upd, steps, diag, ws, st));
+  return DDC_OK;
+}
+
 }  // namespace ddc
 
 using namespace ddc;
@@ -532,6 +563,14 @@ int svd_gpu_backward(int m, int n, const double* U, int ldu, const double* sigma
   g_err[0] = 0;
   return backward_dev(m, n, U, ldu, sigma, V, ldv, gU, ldgu, gsigma, gV, ldgv, gA, ldga,
                       gap_guard, (hipStream_t)stream);
+}
+
+int svd_gpu_mixed(int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
+                  double* dV, int ldv, int update, int refine_steps, double* diag, void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return mixed_dev(n, dA, lda, dsigma, dU, ldu, dV, ldv, update, refine_steps, diag,
+                   (hipStream_t)stream);
 }
 
 int svd_gpu_last_timings(double* ms, int count) {

@@ -91,6 +91,23 @@ void bidiag_plan(ArenaPlan& p, int m, int n, int lda);
 int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
                Arena& ws, hipStream_t st);
 
+// The same with the matrix and its reflectors stored in FP32 (vector
+// arithmetic stays FP64) and the blocked trailing update in reduced
+// precision on the matrix cores: UPD_F32 (FP32 GEMM), UPD_BF16X3 (BF16
+// GEMM on the two-term BF16 split, three products), UPD_BF16 (one BF16
+// product); FP32 accumulation in every case.
+enum { UPD_F32 = 1, UPD_BF16X3 = 2, UPD_BF16 = 3 };
+void bidiag_plan_f32(ArenaPlan& p, int m, int n, int lda, int upd);
+int bidiag_run_f32(int m, int n, float* A, int lda, double* alpha, double* beta, int upd,
+                   Arena& ws, hipStream_t st);
+
+// Mixed-precision SVD of the n x n FP64 matrix A (device pointers): FP32
+// bidiagonalization with update mode upd, FP64 bidiagonal solve and
+// back-transformation, `steps` FP64 refinement steps.  diag: see mixed.hip.
+void mixed_plan(ArenaPlan& p, int n, int upd);
+int mixed_run(int n, const double* A, int lda, double* sigma, double* U, int ldu, double* V,
+              int ldv, int upd, int steps, double* diag, Arena& ws, hipStream_t st);
+
 // Singular values (ascending) of the n x (n+1) upper bidiagonal with diagonal
 // a[n] and super-diagonal b[n] by divide and conquer.
 void dc_plan(ArenaPlan& p, int n);

@@ -1,0 +1,272 @@
+// mixed.hip -- the mixed-precision SVD (svd_gpu_mixed), square matrices.
+//
+// No reference counterpart (the reference computes in FP64 only,
+// svd_gpu.c:53).  Stages:
+//   1. A (FP64) -> FP32 copy; bidiagonalization with the matrix and its
+//      reflectors stored in FP32 (halving the GEMV traffic, which bounds the
+//      plain path), vector arithmetic in FP64, and the blocked trailing
+//      update in FP32 or BF16 on the matrix cores with FP32 accumulation;
+//   2. singular values and bidiagonal vectors in FP64 (divide and conquer,
+//      twisted factorizations), back-transformation in FP64 with the FP32
+//      reflectors widened to FP64;
+//   3. refinement steps in FP64 (Ogita and Aishima, "Iterative refinement
+//      for singular value decomposition based on matrix multiplication",
+//      J. Comput. Appl. Math. 369 (2020)): with T = U^T A V, R = I - U^T U,
+//      S = I - V^T V,
+//         sigma_i = T_ii / (1 - (R_ii + S_ii) / 2)
+//      and for i != j the 2 x 2 system of the first-order conditions
+//         F + F^T = R,  G + G^T = S,  T + F^T Sigma + Sigma G = Sigma
+//      gives F_ij, G_ij; then U <- U (I + F), V <- V (I + G).
+//      Each step is six FP64 GEMMs (library, MFMA) and two small kernels.
+#include <algorithm>
+#include <vector>
+
+#include "ddc_common.h"
+
+namespace ddc {
+
+constexpr double OA_MAXCORR = 1e-3;  // largest first-order correction applied
+
+// B (rows x cols, ld ldb, float) = A (double); rows rows..ldb-1 are zeroed.
+__global__ void mx_to_f32(int rows, int cols, const double* A, int lda, float* B, int ldb) {
+  const int64_t total = (int64_t)ldb * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / ldb, i = e - j * ldb;
+    B[e] = i < rows ? (float)A[i + j * lda] : 0.0f;
+  }
+}
+
+__global__ void mx_to_f64(int64_t count, const float* A, double* B) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x)
+    B[e] = (double)A[e];
+}
+
+// Partial sums of (delta_ij - M_ij)^2 over the n x n matrix M (ld n), one
+// per work-group, in a fixed order.
+__global__ __launch_bounds__(256) void mx_defect(int n, const double* M, double* part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int64_t j = e / n, i = e - j * n;
+    const double d = (i == j ? 1.0 : 0.0) - M[e];
+    s += d * d;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// sigma_i = T_ii / (1 - (r_ii + s_ii)/2) with r = I - U^T U, s = I - V^T V
+// (P = U^T U, Q = V^T V given).
+__global__ void oa_sigma(int n, const double* T, const double* P, const double* Q, double* sig) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = (int64_t)i * n + i;
+  sig[i] = T[d] / (0.5 * (P[d] + Q[d]));
+}
+
+// F and G of the refinement (n x n, ld n).  Off the diagonal:
+//   a = -t_ij - r_ij s_j,  b = -t_ji - s_j q_ij,  d = s_j^2 - s_i^2
+//   f_ij = (-s_j a - s_i b) / d,  g_ij = (-s_i a - s_j b) / d
+// with r = I - U^T U, q = I - V^T V; on it f_ii = r_ii / 2, g_ii = q_ii / 2.
+// Pairs closer than the guard, and pairs whose correction is not small
+// (|f_ij| or |g_ij| > OA_MAXCORR: the vectors were not resolved by the
+// reduced-precision stage and a first-order step would only mix them), get
+// the orthogonalising half alone.
+__global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
+                      const double* sig, double guard2, double* F, double* G) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / n), i = (int)(e - (int64_t)j * n);
+    const double rij = (i == j ? 1.0 : 0.0) - P[e];
+    const double qij = (i == j ? 1.0 : 0.0) - Q[e];
+    if (i == j) {
+      F[e] = 0.5 * rij;
+      G[e] = 0.5 * qij;
+      continue;
+    }
+    const double si = sig[i], sj = sig[j];
+    const double d = (sj - si) * (sj + si);
+    if (!(fabs(d) > guard2)) {
+      F[e] = 0.5 * rij;
+      G[e] = 0.5 * qij;
+      continue;
+    }
+    const double tij = T[e], tji = T[j + (int64_t)i * n];
+    const double a = -tij - rij * sj;
+    const double b = -tji - sj * qij;
+    const double f = (-sj * a - si * b) / d, g = (-si * a - sj * b) / d;
+    // the same pair seen from (j, i): the decision must agree on both sides
+    // so that F + F^T = R and G + G^T = S hold for every pair
+    const double a2 = -tji - rij * si;
+    const double b2 = -tij - si * qij;
+    const double f2 = (-si * a2 - sj * b2) / (-d), g2 = (-sj * a2 - si * b2) / (-d);
+    const bool ok = fabs(f) <= OA_MAXCORR && fabs(g) <= OA_MAXCORR &&
+                    fabs(f2) <= OA_MAXCORR && fabs(g2) <= OA_MAXCORR;
+    F[e] = ok ? f : 0.5 * rij;
+    G[e] = ok ? g : 0.5 * qij;
+  }
+}
+
+struct MixedBufs {
+  int lda_p, ldb;
+  float* A32;
+  double *Ap, *alpha, *beta, *sigma, *Vt, *Ut;
+  double *T, *P, *Q, *W, *F, *G, *U2, *V2, *part;
+};
+
+constexpr int MX_PARTS = 1024;
+
+void mixed_plan(ArenaPlan& p, int n, int upd) {
+  const int lda_p = (int)round_up(n, 16);
+  const int ldb = (int)round_up(n, 16);
+  p.add<float>((size_t)lda_p * n);
+  p.add<double>((size_t)lda_p * n);
+  for (int i = 0; i < 3; ++i) p.add<double>(n);
+  p.add<double>((size_t)ldb * n * 2);
+  for (int i = 0; i < 8; ++i) p.add<double>((size_t)n * n);
+  p.add<double>(MX_PARTS);
+  bidiag_plan_f32(p, n, n, lda_p, upd);
+  dc_plan(p, n);
+  twisted_plan(p, n);
+  backtrans_plan(p, n, n);
+}
+
+static double host_defect(int n, const double* M, double* part, hipStream_t st, int& rc) {
+  hipLaunchKernelGGL(mx_defect, dim3(MX_PARTS), dim3(256), 0, st, n, M, part);
+  std::vector<double> h(MX_PARTS);
+  rc = DDC_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(h.data(), part, sizeof(double) * MX_PARTS, hipMemcpyDeviceToHost, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    rc = DDC_E_HIP;
+    return 0.0;
+  }
+  double s = 0.0;
+  for (double v : h) s += v;
+  return sqrt(s);
+}
+
+// diag (host, 8 entries, may be null): [0] ||I - U^T U||_F and [1]
+// ||I - V^T V||_F before the first refinement step, [2] bidiagonalization,
+// [3] values and vectors, [4] back-transformation, [5] refinement, [6] the
+// FP32 conversion, in milliseconds; [7] largest |change of sigma| / sigma_max
+// in the last refinement step.
+int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int ldu, double* V,
+              int ldv, int upd, int steps, double* diag, Arena& ws, hipStream_t st) {
+  MixedBufs b;
+  b.lda_p = (int)round_up(n, 16);
+  b.ldb = (int)round_up(n, 16);
+  b.A32 = ws.take<float>((size_t)b.lda_p * n);
+  b.Ap = ws.take<double>((size_t)b.lda_p * n);
+  b.alpha = ws.take<double>(n);
+  b.beta = ws.take<double>(n);
+  b.sigma = ws.take<double>(n);
+  b.Vt = ws.take<double>((size_t)b.ldb * n * 2);
+  b.Ut = b.Vt ? b.Vt + (size_t)b.ldb * n : nullptr;
+  double** mats[8] = {&b.T, &b.P, &b.Q, &b.W, &b.F, &b.G, &b.U2, &b.V2};
+  for (auto pm : mats) *pm = ws.take<double>((size_t)n * n);
+  b.part = ws.take<double>(MX_PARTS);
+  if (!b.part) return DDC_E_NOMEM;
+
+  hipEvent_t ev[7];
+  for (auto& e : ev) DDC_HIP(hipEventCreate(&e));
+  struct Ev {
+    hipEvent_t* e;
+    ~Ev() {
+      for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } evg{ev};
+  DDC_HIP(hipEventRecord(ev[0], st));
+  const int64_t tot = (int64_t)b.lda_p * n;
+  const unsigned gconv = (unsigned)std::min<int64_t>(ceil_div(tot, 256), 8192);
+  hipLaunchKernelGGL(mx_to_f32, dim3(gconv), dim3(256), 0, st, n, n, A, lda, b.A32, b.lda_p);
+  DDC_LAUNCHED();
+  DDC_HIP(hipEventRecord(ev[1], st));
+  DDC_TRY(bidiag_run_f32(n, n, b.A32, b.lda_p, b.alpha, b.beta, upd, ws, st));
+  DDC_HIP(hipEventRecord(ev[2], st));
+  DDC_TRY(dc_run(n, b.alpha, b.beta, b.sigma, ws, st));
+  DDC_TRY(twisted_run(n, b.alpha, b.beta, b.sigma, b.Vt, b.Ut, b.ldb, ws, st));
+  DDC_HIP(hipEventRecord(ev[3], st));
+  hipLaunchKernelGGL(mx_to_f64, dim3(gconv), dim3(256), 0, st, tot, b.A32, b.Ap);
+  DDC_LAUNCHED();
+  double* Uc = steps > 0 ? b.U2 : U;
+  double* Vc = steps > 0 ? b.V2 : V;
+  const int lduc = steps > 0 ? n : ldu, ldvc = steps > 0 ? n : ldv;
+  DDC_TRY(backtrans_run(n, n, b.Ap, b.lda_p, b.Ut, b.Vt, b.ldb, Uc, lduc, Vc, ldvc, ws, st));
+  DDC_HIP(hipEventRecord(ev[4], st));
+
+  double defect_u = 0.0, defect_v = 0.0, dsig = 0.0;
+  std::vector<double> s_old(n), s_new(n);
+  for (int it = 0; it < steps; ++it) {
+    // U, V (ld n) in U2 / V2 on entry
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, A, lda, b.V2, n, 0.0, b.W, n, st));
+    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.U2, n, b.W, n, 0.0, b.T, n, st));
+    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.U2, n, b.U2, n, 0.0, b.P, n, st));
+    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.V2, n, b.V2, n, 0.0, b.Q, n, st));
+    if (it == 0 && diag) {
+      int rc;
+      defect_u = host_defect(n, b.P, b.part, st, rc);
+      DDC_TRY(rc);
+      defect_v = host_defect(n, b.Q, b.part, st, rc);
+      DDC_TRY(rc);
+    }
+    DDC_HIP(hipMemcpyAsync(s_old.data(), b.sigma, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(oa_sigma, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n, b.T, b.P,
+                       b.Q, b.sigma);
+    DDC_LAUNCHED();
+    DDC_HIP(hipMemcpyAsync(s_new.data(), b.sigma, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipStreamSynchronize(st));
+    double smax = 0.0;
+    for (double v : s_new) smax = std::max(smax, fabs(v));
+    dsig = 0.0;
+    for (int i = 0; i < n; ++i) dsig = std::max(dsig, fabs(s_new[i] - s_old[i]));
+    dsig = smax > 0 ? dsig / smax : 0.0;
+    const double g = 1e-13 * smax * smax;
+    const unsigned gfg = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 16384);
+    hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, b.T, b.P, b.Q, b.sigma, g, b.F,
+                       b.G);
+    DDC_LAUNCHED();
+    // U <- U + U F, V <- V + V G (new values into W / T, then back)
+    const bool last = it + 1 == steps;
+    double* Un = last ? U : b.W;
+    double* Vn = last ? V : b.T;
+    const int ldun = last ? ldu : n, ldvn = last ? ldv : n;
+    DDC_HIP(hipMemcpy2DAsync(Un, sizeof(double) * ldun, b.U2, sizeof(double) * n,
+                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+    DDC_HIP(hipMemcpy2DAsync(Vn, sizeof(double) * ldvn, b.V2, sizeof(double) * n,
+                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, b.U2, n, b.F, n, 1.0, Un, ldun, st));
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, b.V2, n, b.G, n, 1.0, Vn, ldvn, st));
+    if (!last) {
+      DDC_HIP(hipMemcpyAsync(b.U2, b.W, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
+                             st));
+      DDC_HIP(hipMemcpyAsync(b.V2, b.T, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
+                             st));
+    }
+  }
+  DDC_HIP(hipMemcpyAsync(sigma_out, b.sigma, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  DDC_HIP(hipEventRecord(ev[5], st));
+  DDC_HIP(hipStreamSynchronize(st));
+  if (diag) {
+    float t[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+    diag[0] = defect_u;
+    diag[1] = defect_v;
+    diag[2] = t[1];
+    diag[3] = t[2];
+    diag[4] = t[3];
+    diag[5] = t[4];
+    diag[6] = t[0];
+    diag[7] = dsig;
+  }
+  return DDC_OK;
+}
+
+}  // namespace ddc

@@ -71,6 +71,8 @@ def load():
         lib.svd_gpu_backward.argtypes = [i, i, v, i, v, v, i, v, i, v, v, i, v, i,
                                          ctypes.c_double, v]
         lib.svd_gpu_backward.restype = i
+        lib.svd_gpu_mixed.argtypes = [i, v, i, v, v, i, v, i, i, i, d, v]
+        lib.svd_gpu_mixed.restype = i
         lib.svd_gpu_last_timings.argtypes = [d, i]
         lib.svd_gpu_last_timings.restype = i
         lib.svd_gpu_profile_kernels.argtypes = [i]
@@ -203,6 +205,31 @@ def svd_backward_torch(U, sigma, V, gU, gsigma, gV, gap_guard: float = 1e-13):
     svd_backward_dev(cm(U), sigma.contiguous(), cm(V), cm(gU), gs, cm(gV), gA, gap_guard,
                      stream)
     return gA
+
+
+MIXED_UPDATES = {"f32": 1, "bf16x3": 2, "bf16": 3}
+MIXED_DIAG_KEYS = ["orth_u_before", "orth_v_before", "bidiag_ms", "values_vectors_ms",
+                   "backtransform_ms", "refine_ms", "convert_ms", "last_sigma_change"]
+
+
+def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1):
+    """svd_gpu_mixed on a square CUDA float64 tensor: (sigma, U, V, diag)."""
+    import torch
+    A = col_major(A)
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise ValueError("svd_gpu_mixed needs a square matrix")
+    sigma = torch.empty(n, dtype=torch.float64, device=A.device)
+    U = col_major(torch.empty((n, n), dtype=torch.float64, device=A.device))
+    V = col_major(torch.empty((n, n), dtype=torch.float64, device=A.device))
+    diag = np.zeros(8)
+    stream = torch.cuda.current_stream(A.device).cuda_stream
+    _check(load().svd_gpu_mixed(n, ctypes.c_void_p(A.data_ptr()), A.stride(1),
+                                ctypes.c_void_p(sigma.data_ptr()), ctypes.c_void_p(U.data_ptr()),
+                                U.stride(1), ctypes.c_void_p(V.data_ptr()), V.stride(1),
+                                MIXED_UPDATES[update], refine_steps, _p(diag),
+                                ctypes.c_void_p(stream)))
+    return sigma, U, V, dict(zip(MIXED_DIAG_KEYS, diag.tolist()))
 
 
 def bidiag(A: np.ndarray):

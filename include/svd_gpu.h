@@ -104,6 +104,32 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
                      int ldga, double gap_guard, void *stream);
 
 /*
+ * Mixed-precision SVD of a square n x n FP64 matrix (no reference
+ * counterpart: the reference computes in FP64 only, svd_gpu.c:53).  The
+ * bidiagonalization stores the matrix and its reflectors in FP32 (vector
+ * arithmetic FP64) and runs its blocked trailing update on the matrix cores
+ * with FP32 accumulation in the chosen precision; the bidiagonal solve and
+ * the back-transformation are FP64; then refine_steps (0..8) FP64
+ * refinement steps (Ogita-Aishima, six GEMMs each) restore accuracy where the
+ * FP32 stage resolved the singular vectors.  Device pointers as in
+ * svd_gpu_dev; results ascending, A not modified.
+ *   update       SVD_GPU_MIXED_F32, _BF16X3 (BF16 products on the two-term
+ *                BF16 split: hi*hi + hi*lo + lo*hi) or _BF16 (one product)
+ *   diag         optional host array of 8: [0] ||I-U^T U||_F, [1] ||I-V^T V||_F
+ *                before refinement, [2] bidiagonalization, [3] values and
+ *                vectors, [4] back-transformation, [5] refinement, [6] FP32
+ *                conversion (ms), [7] max |sigma change| / sigma_max in the
+ *                last refinement step
+ * Synchronous: returns after the results are complete.
+ */
+#define SVD_GPU_MIXED_F32 1
+#define SVD_GPU_MIXED_BF16X3 2
+#define SVD_GPU_MIXED_BF16 3
+int svd_gpu_mixed(int n, const double *dA, int lda, double *dsigma, double *dU,
+                  int ldu, double *dV, int ldv, int update, int refine_steps,
+                  double *diag, void *stream);
+
+/*
  * Batched SVD of many small matrices of uneven shapes (no reference
  * counterpart: the reference factors one matrix per call, svd_gpu.c:53).
  * One persistent launch, replayed from a captured HIP graph, factors every
