@@ -87,18 +87,18 @@ __device__ __forceinline__ double sum_partials(const double* part, int cnt) {
 }
 
 // sum_{q<cnt} p[q*stride] in a fixed order, with the loads issued in
-// batches of 8 so that their latencies overlap.
+// batches of 32 (clamped addresses, unused past cnt) so that their
+// latencies overlap.
 __device__ __forceinline__ double sum_strided(const double* p, int cnt, int64_t stride) {
   double acc = 0.0;
-  int q = 0;
-  for (; q + 8 <= cnt; q += 8) {
-    double v[8];
+  for (int q = 0; q < cnt; q += 32) {
+    double v[32];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(q + u) * stride];
+    for (int u = 0; u < 32; ++u) v[u] = p[(int64_t)min(q + u, cnt - 1) * stride];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += v[u];
+    for (int u = 0; u < 32; ++u)
+      if (q + u < cnt) acc += v[u];
   }
-  for (; q < cnt; ++q) acc += p[(int64_t)q * stride];
   return acc;
 }
 
@@ -139,7 +139,8 @@ __global__ __launch_bounds__(256) void bd_init(int m, const TS* A,
 }
 
 // left_gemv: grid.x = ncb column blocks + dot blocks, grid.y = row chunks.
-template <typename TS>
+// NQ: pairs of rows per thread (rows per work-group 512 NQ).
+template <typename TS, int NQ>
 __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int rb0) {
   __shared__ double red[4][CBL];
   __shared__ double s_scal;
@@ -153,11 +154,11 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
   Refl R;
   R.make(s_scal, a.cvec[k]);
 
-  // rows of this thread: r0 + 512 q + {0, 1}, q = 0..3 (coalesced double2)
-  const int r0 = rb0 + blockIdx.y * RC + 2 * tid;
-  double cv[8];
+  // rows of this thread: r0 + 512 q + {0, 1}, q < NQ (coalesced pairs)
+  const int r0 = rb0 + blockIdx.y * (512 * NQ) + 2 * tid;
+  double cv[2 * NQ];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NQ; ++q) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int r = r0 + 512 * q + h;
@@ -166,9 +167,9 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
   }
 
   if ((int)blockIdx.x < ncb) {
-    double u[8];
+    double u[2 * NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int r = r0 + 512 * q + h;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
       if (c < jn) {
         const TS* col = a.A + (int64_t)(j0 + c) * a.lda;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
           const int r = r0 + 512 * q;
           if (r < a.lda) {
             const double2 v = ld2(col + r);
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
     // the reflector itself goes to its final place, column k
     if (blockIdx.x == 0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = r0 + 512 * q + h;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
         if (s < a.t) {
           const TS* col = a.A + (int64_t)(a.p + s) * a.lda;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < NQ; ++q) {
             const int r = r0 + 512 * q;
             if (r < a.lda) {
               const double2 v = ld2(col + r);
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
         } else {
           const double* col = a.X + (int64_t)(s - a.t) * a.lda;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < NQ; ++q) {
             const int r = r0 + 512 * q;
             if (r < a.lda) {
               const double2 v = ld2(col + r);
@@ -287,21 +288,25 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
     double r = (double)a.A[k + (int64_t)j * a.lda];
     const double* Yj = a.Y + j;
     const double* Vj = a.V + j;
-    for (int s0 = 0; s0 < t; s0 += 8) {
-      double ys[8], vs[8];
+    // all panel entries of this column are loaded before the first use, so
+    // the step pays one memory latency instead of one per group of 8
+    double ys[NB], vs[NB];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int s = min(s0 + u, t - 1);  // clamped: in bounds, unused past t
-        ys[u] = Yj[(int64_t)s * a.ldy];
-        vs[u] = Vj[(int64_t)s * a.ldy];
-      }
+    for (int g = 0; g < NB / 8; ++g) {
+      if (g * 8 < t) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int s = s0 + u;
-        if (s < t) {
-          y -= ys[u] * dd[s] + vs[u] * dd[t + s];
-          r -= uk[s] * ys[u] + xk[s] * vs[u];
+        for (int u = 0; u < 8; ++u) {
+          const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
+          ys[g * 8 + u] = Yj[(int64_t)s * a.ldy];
+          vs[g * 8 + u] = Vj[(int64_t)s * a.ldy];
         }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      if (s < t) {
+        y -= ys[s] * dd[s] + vs[s] * dd[t + s];
+        r -= uk[s] * ys[s] + xk[s] * vs[s];
       }
     }
     y *= 2.0;
@@ -315,9 +320,10 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
 }
 
 // right_gemv: grid.x = row blocks + dot blocks, grid.y = column chunks.
-template <typename TS>
+// CC: trailing columns per work-group.
+template <typename TS, int CC>
 __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0) {
-  __shared__ double vs[CCR];
+  __shared__ double vs[CC];
   __shared__ double red[4][DPW];
   __shared__ double s_scal;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -329,8 +335,8 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   __syncthreads();
   Refl R;
   R.make(s_scal, a.rvec[k + 1]);
-  const int j0 = k + 1 + blockIdx.y * CCR;
-  const int jn = min(CCR, a.n - j0);
+  const int j0 = k + 1 + blockIdx.y * CC;
+  const int jn = min(CC, a.n - j0);
   if (tid < jn) vs[tid] = R.vec(a.rvec[j0 + tid], j0 + tid == k + 1);
   __syncthreads();
 
@@ -431,24 +437,27 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
     if (!noright) x = sum_strided(a.Rpart + row, nccR, a.lda);
     const TS* Ur = a.A + row + (int64_t)a.p * a.lda;
     const double* Xr = a.X + row;
-    for (int s0 = 0; s0 < t; s0 += 8) {
-      double us[8], xs[8];
+    // all panel entries of this row are loaded before the first use
+    double us[NB], xs[NB];
+    const double ut = (double)Ur[(int64_t)t * a.lda];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int s = min(s0 + u, t - 1);  // clamped: in bounds, unused past t
-        us[u] = (double)Ur[(int64_t)s * a.lda];
-        xs[u] = Xr[(int64_t)s * a.lda];
-      }
+    for (int g = 0; g < NB / 8; ++g) {
+      if (g * 8 < t) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int s = s0 + u;
-        if (s < t) {
-          if (!noright) x -= us[u] * dyv[s] + xs[u] * dyv[t + 1 + s];
-          c -= us[u] * yk1[s] + xs[u] * vk1[s];
+        for (int u = 0; u < 8; ++u) {
+          const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
+          us[g * 8 + u] = (double)Ur[(int64_t)s * a.lda];
+          xs[g * 8 + u] = Xr[(int64_t)s * a.lda];
         }
       }
     }
-    double ut = (double)Ur[(int64_t)t * a.lda];
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+      if (s < t) {
+        if (!noright) x -= us[s] * dyv[s] + xs[s] * dyv[t + 1 + s];
+        c -= us[s] * yk1[s] + xs[s] * vk1[s];
+      }
+    }
     if (!noright) {
       x -= ut * dyv[t];
       x *= 2.0;
@@ -649,11 +658,12 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     }
     // left reflector of column k and the left GEMV
     const int rb0 = (k / 64) * 64;
-    const int nchL = (int)ceil_div(m - rb0, RC);
     const int ncb = (int)ceil_div(n - k - 1, CBL);
     const int ndl = (int)ceil_div(2 * a.t, DPW);
+    const int nchL = (int)ceil_div(m - rb0, RC);
     const int kl0 = kclock_on() ? kclock_mark(st) : -1;
-    hipLaunchKernelGGL(bd_left_gemv<TS>, dim3(ncb + ndl, nchL), dim3(256), 0, st, a, ncb, rb0);
+    hipLaunchKernelGGL((bd_left_gemv<TS, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0, st, a,
+                       ncb, rb0);
     DDC_LAUNCHED();
     if (kl0 >= 0) kclock_span(KCLOCK_LEFT_GEMV, kl0, kclock_mark(st));
     const int nfl = (int)ceil_div(n - k - 1, FIN);
@@ -668,10 +678,11 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     } else {
       const int rbr = ((k + 1) / 64) * 64;
       const int nrb = (int)ceil_div(m - rbr, RBR);
-      const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
+      const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int kr0 = kclock_on() ? kclock_mark(st) : -1;
-      hipLaunchKernelGGL(bd_right_gemv<TS>, dim3(nrb + ndr, ncc), dim3(256), 0, st, a, nrb, rbr);
+      hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + ndr, ncc), dim3(256), 0, st, a,
+                         nrb, rbr);
       DDC_LAUNCHED();
       if (kr0 >= 0) kclock_span(KCLOCK_RIGHT_GEMV, kr0, kclock_mark(st));
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
