@@ -162,4 +162,42 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   return DDC_OK;
 }
 
+// C (rows x cols, ld ldc) = the leading part of the identity.
+__global__ void bt_identity(int rows, int cols, double* C, int ldc) {
+  const int64_t total = (int64_t)rows * cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / rows, i = e - j * rows;
+    C[i + j * ldc] = i == j ? 1.0 : 0.0;
+  }
+}
+
+void form_q_plan(ArenaPlan& p, int m, int n, bool left) {
+  const int ncol = left ? m : n;
+  p.add<double>((size_t)NBB * NBB * 2);
+  p.add<double>((size_t)NBB * ncol);
+  p.add<double>((size_t)(left ? m : n) * NBB);
+}
+
+// Q = H_0 ... H_{n-1} (left, m x m) or G_0 ... G_{n-3} (right, n x n) from the
+// reflectors stored in A (m x n, m >= n), as form_u_par / form_v_par build
+// them column by column (bidiag_par.c:877-988); here one compact-WY block at
+// a time on the identity.
+int form_q_run(bool left, int m, int n, const double* A, int lda, double* Q, int ldq, Arena& ws,
+               hipStream_t st) {
+  if (m < n || n < 1) return DDC_E_ARG;
+  const int k = left ? m : n;
+  double* G = ws.take<double>((size_t)NBB * NBB * 2);
+  double* Ti = G ? G + NBB * NBB : nullptr;
+  double* Z = ws.take<double>((size_t)NBB * k);
+  double* Wd = ws.take<double>((size_t)k * NBB);
+  if (!Wd) return DDC_E_NOMEM;
+  const unsigned g = (unsigned)std::min<int64_t>(ceil_div((int64_t)k * k, 256), 8192);
+  hipLaunchKernelGGL(bt_identity, dim3(g), dim3(256), 0, st, k, k, Q, ldq);
+  DDC_LAUNCHED();
+  if (left) return apply_blocks(true, m, n, A, lda, n, Q, ldq, k, G, Ti, Z, Wd, st);
+  if (n > 2) return apply_blocks(false, m, n, A, lda, n - 2, Q, ldq, k, G, Ti, Z, Wd, st);
+  return DDC_OK;
+}
+
 }  // namespace ddc

@@ -468,6 +468,39 @@ static int backward_dev(int m, int n, const double* U, int ldu, const double* si
   return DDC_OK;
 }
 
+// host buffers: A (m x n, reflectors as bidiag_gpu leaves them) -> Q
+static int form_host(bool left, int m, int n, const double* A, double* Q) {
+  DDC_TRY(check_dims(m, n));
+  if (m < n) {
+    set_err("bidiag_form_u_gpu / bidiag_form_v_gpu require m >= n");
+    return DDC_E_ARG;
+  }
+  if (!A || !Q) {
+    set_err("null pointer argument");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = g_ctx.st;
+  const int lda_p = (int)round_up(m, 16);
+  const int k = left ? m : n;
+  ArenaPlan plan;
+  plan.add<double>((size_t)lda_p * n);
+  plan.add<double>((size_t)k * k);
+  form_q_plan(plan, m, n, left);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  double* Ap = ws.take<double>((size_t)lda_p * n);
+  double* Qd = ws.take<double>((size_t)k * k);
+  if (!Qd) return DDC_E_NOMEM;
+  DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
+  DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, A, sizeof(double) * m,
+                           sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+  DDC_TRY(form_q_run(left, m, n, Ap, lda_p, Qd, k, ws, st));
+  DDC_HIP(hipMemcpyAsync(Q, Qd, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  return DDC_OK;
+}
+
 static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
                      double* dV, int ldv, int upd, int steps, double* diag, hipStream_t user) {
   DDC_TRY(check_dims(n, n));
@@ -571,6 +604,18 @@ int svd_gpu_mixed(int n, const double* dA, int lda, double* dsigma, double* dU, 
   g_err[0] = 0;
   return mixed_dev(n, dA, lda, dsigma, dU, ldu, dV, ldv, update, refine_steps, diag,
                    (hipStream_t)stream);
+}
+
+int bidiag_form_u_gpu(int m, int n, const double* A, double* U) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return form_host(true, m, n, A, U);
+}
+
+int bidiag_form_v_gpu(int m, int n, const double* A, double* V) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return form_host(false, m, n, A, V);
 }
 
 int svd_gpu_last_timings(double* ms, int count) {

@@ -130,6 +130,12 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
                   const double* Vt, int ldb, double* U, int ldu, double* V,
                   int ldv, Arena& ws, hipStream_t st);
 
+// Explicit Q = H_0 ... H_{n-1} (left, m x m) or P = G_0 ... G_{n-3} (right,
+// n x n) from the reflectors stored in A (m >= n).
+void form_q_plan(ArenaPlan& p, int m, int n, bool left);
+int form_q_run(bool left, int m, int n, const double* A, int lda, double* Q, int ldq, Arena& ws,
+               hipStream_t st);
+
 // Reverse-mode derivative of the thin SVD (m >= n): Abar from U, sigma, V and
 // the cotangents (null = zero).  Device pointers.
 void adjoint_plan(ArenaPlan& p, int m, int n);

@@ -64,6 +64,10 @@ def load():
         lib.svd_gpu_dev.restype = i
         lib.bidiag_gpu.argtypes = [i, i, d, d, d]
         lib.bidiag_gpu.restype = i
+        lib.bidiag_form_u_gpu.argtypes = [i, i, d, d]
+        lib.bidiag_form_u_gpu.restype = i
+        lib.bidiag_form_v_gpu.argtypes = [i, i, d, d]
+        lib.bidiag_form_v_gpu.restype = i
         lib.bdsvd_values_gpu.argtypes = [i, d, d, d, i]
         lib.bdsvd_values_gpu.restype = i
         lib.bdsvd_vectors_gpu.argtypes = [i, d, d, d, d, d]
@@ -240,6 +244,24 @@ def bidiag(A: np.ndarray):
     beta = np.zeros(max(n - 1, 1))
     _check(load().bidiag_gpu(m, n, _p(A), _p(alpha), _p(beta)))
     return A, alpha, beta[: n - 1]
+
+
+def form_u(A_mod: np.ndarray) -> np.ndarray:
+    """bidiag_form_u_gpu: U (m x m) from the reflectors left by bidiag()."""
+    A_mod = np.array(A_mod, dtype=np.float64, order="F", copy=True)
+    m, n = A_mod.shape
+    U = np.zeros((m, m), order="F")
+    _check(load().bidiag_form_u_gpu(m, n, _p(A_mod), _p(U)))
+    return U
+
+
+def form_v(A_mod: np.ndarray) -> np.ndarray:
+    """bidiag_form_v_gpu: V (n x n) from the reflectors left by bidiag()."""
+    A_mod = np.array(A_mod, dtype=np.float64, order="F", copy=True)
+    m, n = A_mod.shape
+    V = np.zeros((n, n), order="F")
+    _check(load().bidiag_form_v_gpu(m, n, _p(A_mod), _p(V)))
+    return V
 
 
 def bdsvd_values(alpha, beta, refine: bool = False) -> np.ndarray:

@@ -68,6 +68,16 @@ int svd_gpu_dev(int m, int n, const double *dA, int lda, double *dsigma,
 int bidiag_gpu(int m, int n, double *A, double *alpha, double *beta);
 
 /*
+ * Explicit orthogonal factors of the bidiagonalization A = U B V^T, the
+ * replacements of form_u_par / form_v_par (bidiag_par.h:33-34,
+ * bidiag_par.c:877-988, equal to form_u / form_v of bidiag.c:252-359).
+ * A (m x n, m >= n, host) holds the reflectors as bidiag_gpu leaves them;
+ * U gets m x m, V gets n x n (host, column-major).
+ */
+int bidiag_form_u_gpu(int m, int n, const double *A, double *U);
+int bidiag_form_v_gpu(int m, int n, const double *A, double *V);
+
+/*
  * Singular values (ascending) of the n x n upper bidiagonal with diagonal
  * alpha[n] and super-diagonal beta[n-1], by the GPU divide and conquer
  * (replacement of GetSingularValues_Parallel, Calculations-Parallel.h:41,

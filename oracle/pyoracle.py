@@ -115,6 +115,33 @@ def ref_bidiag(A):
     return A, alpha, beta[:len_beta]
 
 
+def _form(lib, fname, A_mod, left: bool):
+    A_mod = np.array(A_mod, dtype=np.float64, order="F", copy=True)
+    m, n = A_mod.shape
+    k = m if left else n
+    Q = np.zeros((k, k), order="F")
+    getattr(lib, fname)(ctypes.c_int(m), ctypes.c_int(n), _ptr(A_mod), _ptr(Q))
+    return Q
+
+
+def oracle_form_u(A_mod):
+    return _form(oracle_lib(), "orc_form_u", A_mod, True)
+
+
+def oracle_form_v(A_mod):
+    return _form(oracle_lib(), "orc_form_v", A_mod, False)
+
+
+def ref_form_u(A_mod):
+    lib = ref_lib()
+    return None if lib is None else _form(lib, "form_u", A_mod, True)
+
+
+def ref_form_v(A_mod):
+    lib = ref_lib()
+    return None if lib is None else _form(lib, "form_v", A_mod, False)
+
+
 def oracle_dc_values(b1, b2):
     b1 = np.ascontiguousarray(b1, dtype=np.float64)
     b2 = np.ascontiguousarray(b2, dtype=np.float64)

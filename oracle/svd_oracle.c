@@ -724,3 +724,41 @@ int orc_svd(int m, int n, double *A, double *sigma, double *U, double *V,
   free(AT); free(alpha); free(beta); free(X); free(Y);
   return 0;
 }
+
+/* form_u, bidiag.c:252-302: column i of U is H_0 ... H_top e_i with
+ * top = min(i, mn-1), the reflectors applied from the last one down. */
+void orc_form_u(int m, int n, const double *A_mod, double *U) {
+  const int mn = m < n ? m : n;
+  for (int i = 0; i < m; ++i) {
+    double *u = U + (size_t)i * m;
+    for (int r = 0; r < m; ++r) u[r] = 0.0;
+    u[i] = 1.0;
+    const int top = i < mn - 1 ? i : mn - 1;
+    for (int j = top; j >= 0; --j) {
+      const double *v = A_mod + j + (size_t)j * m;
+      double d = 0.0;
+      for (int r = 0; r < m - j; ++r) d += u[j + r] * v[r];
+      for (int r = 0; r < m - j; ++r) u[j + r] -= 2.0 * v[r] * d;
+    }
+  }
+}
+
+/* form_v, bidiag.c:305-359: column i of V is G_0 ... G_top e_i, the right
+ * reflector j being row j of A_mod from column j+1 on, num_refs = mn-1 for
+ * m >= n (mn for m < n), top = min(i-1, num_refs-1). */
+void orc_form_v(int m, int n, const double *A_mod, double *V) {
+  const int mn = m < n ? m : n;
+  const int num_refs = m < n ? mn : mn - 1;
+  for (int i = 0; i < n; ++i) {
+    double *v = V + (size_t)i * n;
+    for (int r = 0; r < n; ++r) v[r] = 0.0;
+    v[i] = 1.0;
+    const int top = i < num_refs ? i - 1 : num_refs - 1;
+    for (int j = top; j >= 0; --j) {
+      const double *w = A_mod + j + (size_t)(j + 1) * m;  /* row j, stride m */
+      double d = 0.0;
+      for (int r = 0; r < n - j - 1; ++r) d += w[(size_t)r * m] * v[j + 1 + r];
+      for (int r = 0; r < n - j - 1; ++r) v[j + 1 + r] -= 2.0 * w[(size_t)r * m] * d;
+    }
+  }
+}

@@ -68,6 +68,13 @@ int orc_svd(int m, int n, double *A, double *sigma, double *U, double *V,
  * [bidiag, values, vectors, back-transformation]. */
 void orc_last_timings(double out[4]);
 
+/* Explicit orthogonal factors of the bidiagonalization from the stored
+ * reflectors: U (m x m) as form_u (bidiag.c:252-302, = form_u_par
+ * bidiag_par.c:877-932) and V (n x n) as form_v (bidiag.c:305-359, =
+ * form_v_par bidiag_par.c:934-988).  A_mod is m x n, leading dimension m. */
+void orc_form_u(int m, int n, const double *A_mod, double *U);
+void orc_form_v(int m, int n, const double *A_mod, double *V);
+
 #ifdef __cplusplus
 }
 #endif

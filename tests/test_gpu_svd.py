@@ -343,3 +343,21 @@ def test_svd_gpu_rejects_nonfinite(ddcsvd):
     assert rc == ddcsvd.E_NUMERIC
     rc, *_ = ddcsvd.svd(np.zeros((0, 3)), raw=True)
     assert rc == ddcsvd.E_ARG
+
+
+@pytest.mark.parametrize("shape", [(200, 120), (128, 128), (300, 1), (2, 2), (1, 1), (520, 300)])
+def test_form_u_v_matches_oracle(ddcsvd, po, shape):
+    """bidiag_form_u_gpu / bidiag_form_v_gpu against the oracle's form_u /
+    form_v applied to the same reflectors, and U^T A V = B."""
+    m, n = shape
+    gpu_case(f"form_u/form_v {m}x{n}")
+    A = gen.uniform_matrix(m, n, seed=70 + m)
+    Am, al, be = ddcsvd.bidiag(A)
+    U, V = ddcsvd.form_u(Am), ddcsvd.form_v(Am)
+    Uo, Vo = po.oracle_form_u(Am), po.oracle_form_v(Am)
+    assert np.max(np.abs(U - Uo)) <= 1e-13 * max(m, 8)
+    assert np.max(np.abs(V - Vo)) <= 1e-13 * max(n, 8)
+    B = np.zeros((m, n))
+    B[np.arange(n), np.arange(n)] = al
+    B[np.arange(n - 1), np.arange(1, n)] = be
+    assert np.linalg.norm(U.T @ A @ V - B) <= 1e-13 * max(1.0, np.linalg.norm(A)) * max(m, 8) ** 0.5

@@ -39,6 +39,32 @@ def test_oracle_bit_equal_to_compiled_reference(po, shape):
         assert np.array_equal(o[k], r[k], equal_nan=True), k
 
 
+def _bidiagonal(m, n, alpha, beta):
+    B = np.zeros((m, n))
+    for i in range(len(alpha)):
+        B[i, i] = alpha[i]
+    for i in range(len(beta)):
+        B[i, i + 1] = beta[i]
+    return B
+
+
+@pytest.mark.parametrize("shape", [(40, 40), (60, 25), (17, 17), (3, 2), (1, 1)])
+def test_oracle_form_u_v(po, shape):
+    """form_u / form_v restated: orthogonal, U^T A V = B (closed form of the
+    reduction), and bit-equal to the compiled reference where it is built."""
+    m, n = shape
+    A = gen.uniform_matrix(m, n, 4 + m)
+    Am, al, be = po.oracle_bidiag(A)
+    U, V = po.oracle_form_u(Am), po.oracle_form_v(Am)
+    B = _bidiagonal(m, n, al, be)
+    assert np.linalg.norm(U.T @ A @ V - B) <= 1e-14 * max(1.0, np.linalg.norm(A)) * max(m, 4)
+    assert np.linalg.norm(U.T @ U - np.eye(m)) <= 1e-14 * max(m, 4)
+    assert np.linalg.norm(V.T @ V - np.eye(n)) <= 1e-14 * max(n, 4)
+    if have_ref(po):
+        assert np.array_equal(U, po.ref_form_u(Am))
+        assert np.array_equal(V, po.ref_form_v(Am))
+
+
 def test_oracle_matches_golden_fixtures(po):
     A = gen.glibc_rand_matrix(512, 512, 1.0, 4.0, seed=1)
     Ao, ao, bo = po.oracle_bidiag(A)
