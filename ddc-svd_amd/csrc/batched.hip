@@ -210,9 +210,53 @@ __host__ __device__ constexpr int pair_q(int i) {
   return KIND == 0 ? B + i : (i / (B / 2)) * B + B - 1 - i % (B / 2);
 }
 
+template <int B>
+struct Log2;
+template <>
+struct Log2<2> {
+  static constexpr int v = 1;
+};
+template <>
+struct Log2<4> {
+  static constexpr int v = 2;
+};
+template <>
+struct Log2<8> {
+  static constexpr int v = 3;
+};
+
+// Wave sum of each g[i], delivered to the lanes of group i (the 64/B lanes
+// with lane >> (6 - log2 B) == i).  The first log2 B butterfly steps halve the
+// set of values a lane carries (it keeps the half its lane bit selects and
+// sends the other half), the remaining steps finish the sum inside the
+// group: (B - 1) + (6 - log2 B) exchanges instead of 6 B.  Every lane of a
+// group ends with the same value.
+template <typename T, int B>
+__device__ __forceinline__ T reduce_scatter(T (&g)[B]) {
+  constexpr int LB = Log2<B>::v;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int L = 0; L < LB; ++L) {
+    const int bit = 5 - L;
+    const int half = B >> (L + 1);
+    const bool upper = (lane >> bit) & 1;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const T send = upper ? g[j] : g[j + half];
+      const T keep = upper ? g[j + half] : g[j];
+      g[j] = keep + __shfl_xor(send, 1 << bit, 64);
+    }
+  }
+  T v = g[0];
+#pragma unroll
+  for (int bit = 5 - LB; bit >= 0; --bit) v += __shfl_xor(v, 1 << bit, 64);
+  return v;
+}
+
 // One round: B disjoint rotations.  nrm: lane c holds the squared norm of
-// the column at position c.  The dot products are reduced across the wave;
-// lane i then computes rotation i, and the cosines and sines are broadcast.
+// the column at position c.  The dot products are reduce-scattered across
+// the wave so that the lanes of group i hold pair i's; each group computes
+// its rotation, and the cosines and sines are broadcast.
 template <typename T, int B, int R, int RV, int KIND>
 __device__ __forceinline__ void jround(Block<T, B, R, RV>& k, T& nrm, bool enI, bool enJ, T tol,
                                        T& off) {
@@ -228,18 +272,14 @@ __device__ __forceinline__ void jround(Block<T, B, R, RV>& k, T& nrm, bool enI, 
     }
     g[i] = d;
   }
-#pragma unroll
-  for (int i = 0; i < B; ++i) g[i] = wsum(g[i]);
-  T gl = g[0];
-#pragma unroll
-  for (int i = 1; i < B; ++i) gl = lane == i ? g[i] : gl;
-  const int li = lane < B ? lane : 0;
+  constexpr int GS = 64 / B;  // lanes per group
+  const T gl = reduce_scatter<T, B>(g);
+  const int li = lane / GS;  // the pair this lane's group handles
   const int pl = KIND == 0 ? li : (li / (B / 2)) * B + li % (B / 2);
   const int ql = KIND == 0 ? B + li : (li / (B / 2)) * B + B - 1 - li % (B / 2);
   const T al = __shfl(nrm, pl, 64), be = __shfl(nrm, ql, 64);
   T c = T(1), s = T(0), t = T(0);
-  bool act = false;
-  if (lane < B) act = jrot(al, be, gl, tol, c, s, t, off);
+  const bool act = jrot(al, be, gl, tol, c, s, t, off);
   const T aln = act ? fmax(al - t * gl, T(0)) : al;
   const T ben = act ? be + t * gl : be;
   // new norms back to the column lanes
@@ -254,12 +294,13 @@ __device__ __forceinline__ void jround(Block<T, B, R, RV>& k, T& nrm, bool enI, 
     slot = h * (B / 2) + (qside ? B - 1 - x : x);
   }
   slot = lane < 2 * B ? slot : 0;
-  const T np = __shfl(aln, slot, 64), nq = __shfl(ben, slot, 64);
+  const T np = __shfl(aln, slot * GS, 64), nq = __shfl(ben, slot * GS, 64);
   if (lane < 2 * B) nrm = qside ? nq : np;
 #pragma unroll
   for (int i = 0; i < B; ++i) {
-    const T si = rdlane(s, i);
-    if (si != T(0)) rotate_cols(k, pair_p<B, KIND>(i), pair_q<B, KIND>(i), rdlane(c, i), si);
+    const T si = rdlane(s, i * GS);
+    if (si != T(0))
+      rotate_cols(k, pair_p<B, KIND>(i), pair_q<B, KIND>(i), rdlane(c, i * GS), si);
   }
 }
 
