@@ -31,12 +31,14 @@ namespace ddc {
 
 constexpr int NB = 32;        // panel width
 constexpr int RC = 2048;      // rows per left_gemv chunk (8 per thread)
-constexpr int CBL = 16;       // columns per left_gemv work-group
+constexpr int CBL = 8;        // columns per left_gemv work-group
 constexpr int RBR = 512;      // rows per right_gemv block (2 per thread)
 constexpr int CCR = 128;      // columns per right_gemv chunk
 constexpr int DPW = 8;        // panel dot products per dot work-group
 constexpr int FIN = 64;       // items per finalize work-group (one wave)
 constexpr int DSTRIDE = 2 * NB + 1;
+// bd_left_gemv's reduction array red[4][CBL] also serves its dot work-groups
+static_assert(CBL >= DPW, "left_gemv: red[][CBL] must hold DPW dot products");
 
 // TS: storage type of the matrix and of the reflectors kept in it (double,
 // or float for the mixed-precision form); all vector arithmetic is FP64.
