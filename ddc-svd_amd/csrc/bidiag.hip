@@ -268,42 +268,49 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
   __shared__ double xk[NB];      // x_s[k], s < t
   const int tid = threadIdx.x;
   const int k = a.k, t = a.t;
-  Refl R;
-  R.make(sum_partials(a.cnorm, a.ncn), a.cvec[k]);
-  if (tid < 2 * t) {
-    double s = sum_strided(a.Ldot + tid, nchL, 2 * NB);
-    double w1 = (tid < t) ? (double)a.A[k + (int64_t)(a.p + tid) * a.lda]
-                          : a.X[k + (int64_t)(tid - t) * a.lda];
-    dd[tid] = R.dot(s, w1);
+  const int j = k + 1 + blockIdx.x * FIN + tid;
+  const bool jv = j < a.n;
+  const int jc = jv ? j : k + 1;  // a valid column for the loads of idle lanes
+
+  // Every load first: none depends on the reflector, so their latencies
+  // overlap instead of following the block's setup.
+  double ys[NB], vs[NB];
+  const double* Yj = a.Y + jc;
+  const double* Vj = a.V + jc;
+#pragma unroll
+  for (int g = 0; g < NB / 8; ++g) {
+    if (g * 8 < t) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
+        ys[g * 8 + u] = Yj[(int64_t)s * a.ldy];
+        vs[g * 8 + u] = Vj[(int64_t)s * a.ldy];
+      }
+    }
   }
+  double r = (double)a.A[k + (int64_t)jc * a.lda];
+  double cn = 0.0;
+  for (int i = tid; i < a.ncn; i += FIN) cn += a.cnorm[i];
+  const double ck = a.cvec[k];
+  const bool dotl = tid < 2 * t;
+  const int td = dotl ? tid : 0;
+  const double w1 = (td < t) ? (double)a.A[k + (int64_t)(a.p + td) * a.lda]
+                             : a.X[k + (int64_t)(td - t) * a.lda];
+  const double ds = sum_strided(a.Ldot + td, nchL, 2 * NB);
+  double y = sum_strided(a.Lpart + jc, nchL, a.ldy);
+
+  Refl R;
+  R.make(wave_sum(cn), ck);
+  if (dotl) dd[tid] = R.dot(ds, w1);
   if (tid < t) {
     uk[tid] = (double)a.A[k + (int64_t)(a.p + tid) * a.lda];
     xk[tid] = a.X[k + (int64_t)tid * a.lda];
   }
-  if (tid == t) uk[t] = R.vec(a.cvec[k], true);
+  if (tid == t) uk[t] = R.vec(ck, true);
   __syncthreads();
 
-  const int j = k + 1 + blockIdx.x * FIN + tid;
   double rr = 0.0;
-  if (j < a.n) {
-    double y = sum_strided(a.Lpart + j, nchL, a.ldy);
-    double r = (double)a.A[k + (int64_t)j * a.lda];
-    const double* Yj = a.Y + j;
-    const double* Vj = a.V + j;
-    // all panel entries of this column are loaded before the first use, so
-    // the step pays one memory latency instead of one per group of 8
-    double ys[NB], vs[NB];
-#pragma unroll
-    for (int g = 0; g < NB / 8; ++g) {
-      if (g * 8 < t) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
-          ys[g * 8 + u] = Yj[(int64_t)s * a.ldy];
-          vs[g * 8 + u] = Vj[(int64_t)s * a.ldy];
-        }
-      }
-    }
+  if (jv) {
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
       if (s < t) {
@@ -414,45 +421,50 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   __shared__ double vk1[NB];       // V[k+1, s], s <= t
   const int tid = threadIdx.x;
   const int k = a.k, t = a.t;
-  Refl R;
-  R.nrm = R.sgn = R.rl = 0.0;
-  if (!noright) {
-    R.make(sum_partials(a.rnorm, a.nrn), a.rvec[k + 1]);
-    if (tid < 2 * t + 1) {
-      double s = sum_strided(a.Rdot + tid, nccR, DSTRIDE);
-      double w1 = (tid <= t) ? a.Y[(k + 1) + (int64_t)tid * a.ldy]
-                             : a.V[(k + 1) + (int64_t)(tid - t - 1) * a.ldy];
-      dyv[tid] = R.dot(s, w1);
-    }
-  }
-  if (tid <= t) yk1[tid] = a.Y[(k + 1) + (int64_t)tid * a.ldy];
-  if (tid < t) vk1[tid] = a.V[(k + 1) + (int64_t)tid * a.ldy];
-  if (tid == t) vk1[t] = noright ? 0.0 : R.vec(a.rvec[k + 1], true);
-  __syncthreads();
-
   const int row = k + 1 + blockIdx.x * FIN + tid;
+  const bool rv = row < a.m;
+  const int rc = rv ? row : k + 1;  // a valid row for the loads of idle lanes
   const bool next = k + 1 < a.n;
-  double cc2 = 0.0;
-  if (row < a.m) {
-    double x = 0.0;
-    double c = next ? (double)a.A[row + (int64_t)(k + 1) * a.lda] : 0.0;
-    if (!noright) x = sum_strided(a.Rpart + row, nccR, a.lda);
-    const TS* Ur = a.A + row + (int64_t)a.p * a.lda;
-    const double* Xr = a.X + row;
-    // all panel entries of this row are loaded before the first use
-    double us[NB], xs[NB];
-    const double ut = (double)Ur[(int64_t)t * a.lda];
+
+  // Every load first (see bd_left_fin).
+  const TS* Ur = a.A + rc + (int64_t)a.p * a.lda;
+  const double* Xr = a.X + rc;
+  double us[NB], xs[NB];
 #pragma unroll
-    for (int g = 0; g < NB / 8; ++g) {
-      if (g * 8 < t) {
+  for (int g = 0; g < NB / 8; ++g) {
+    if (g * 8 < t) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
-          us[g * 8 + u] = (double)Ur[(int64_t)s * a.lda];
-          xs[g * 8 + u] = Xr[(int64_t)s * a.lda];
-        }
+      for (int u = 0; u < 8; ++u) {
+        const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
+        us[g * 8 + u] = (double)Ur[(int64_t)s * a.lda];
+        xs[g * 8 + u] = Xr[(int64_t)s * a.lda];
       }
     }
+  }
+  const double ut = (double)Ur[(int64_t)t * a.lda];
+  double c = next ? (double)a.A[rc + (int64_t)(k + 1) * a.lda] : 0.0;
+  double rn = 0.0;
+  if (!noright)
+    for (int i = tid; i < a.nrn; i += FIN) rn += a.rnorm[i];
+  const double rk1 = a.rvec[k + 1];
+  const bool dotr = !noright && tid < 2 * t + 1;
+  const int td = dotr ? tid : 0;
+  const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
+                              : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
+  const double ds = noright ? 0.0 : sum_strided(a.Rdot + td, nccR, DSTRIDE);
+  double x = noright ? 0.0 : sum_strided(a.Rpart + rc, nccR, a.lda);
+
+  Refl R;
+  R.nrm = R.sgn = R.rl = 0.0;
+  if (!noright) R.make(wave_sum(rn), rk1);
+  if (dotr) dyv[tid] = R.dot(ds, w1);
+  if (tid <= t) yk1[tid] = a.Y[(k + 1) + (int64_t)tid * a.ldy];
+  if (tid < t) vk1[tid] = a.V[(k + 1) + (int64_t)tid * a.ldy];
+  if (tid == t) vk1[t] = noright ? 0.0 : R.vec(rk1, true);
+  __syncthreads();
+
+  double cc2 = 0.0;
+  if (rv) {
 #pragma unroll
     for (int s = 0; s < NB; ++s) {
       if (s < t) {
