@@ -21,7 +21,7 @@
 
 namespace ddc {
 
-constexpr int NBB = 256;  // reflectors per compact-WY block
+constexpr int NBB = 512;  // reflectors per compact-WY block
 
 // C(0:n, 0:n) = Bt^T where Bt[j*ldb + i] holds vector i, entry j;
 // rows n..rows-1 of C are zeroed.
