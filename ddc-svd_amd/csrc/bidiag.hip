@@ -10,17 +10,20 @@
 // (u_s / v_s: left / right reflectors of the panel, y_s = 2 M^T u_s,
 // x_s = 2 M v_s).  A step t (global column k) is four stream-ordered kernels:
 //
-//   left_gemv   g = A0[k:m, k+1:n]^T u           2-D grid, partial sums
-//   left_fin    y_t = 2(g - Y (U^T u) - V (X^T u)); row k of M -> r
-//   right_gemv  h = A0[k+1:m, k+1:n] v           2-D grid, partial sums
-//   right_fin   x_t = 2(h - U (Y^T v) - X (V^T v)); column k+1 of M -> c
+//   left_gemv   g = A0[k:m, k+1:n]^T c           2-D grid, partial sums
+//   left_fin    A0^T u = (g + s|c| A0[k, :]) / rl;
+//               y_t = 2(A0^T u - Y (U^T u) - V (X^T u)); row k of M -> r
+//   right_gemv  h = A0[k+1:m, k+1:n] r           2-D grid, partial sums
+//   right_fin   A0 v = (h + s|r| A0[:, k+1]) / rl;
+//               x_t = 2(A0 v - U (Y^T v) - X (V^T v)); column k+1 of M -> c
 //
-// The reflector of the current column/row is never materialised before it
-// is used: each consumer rebuilds u = (c + s|c| e_k)/rl from c and the
-// partial sums of |c|^2 left by the producer (every work-group redoes that
-// tiny fixed-order reduction, so no grid-wide synchronisation is needed and
-// results are bitwise reproducible).  The two GEMVs are the only passes
-// over the trailing matrix: 16 bytes per trailing element per step.  After
+// The GEMVs multiply by the raw column c / row r of M, so their loads wait
+// for nothing; the reflector u = (c + s|c| e_k)/rl (v likewise) enters in
+// the fin kernels, which rebuild |c|^2 from the partial sums left by the
+// producer (every work-group redoes that tiny fixed-order reduction, so no
+// grid-wide synchronisation is needed and results are bitwise
+// reproducible).  The two GEMVs are the only passes over the trailing
+// matrix: 16 bytes per trailing element per step.  After
 // NB steps the trailing matrix gets A0 -= [U X] [Y V]^T, one FP64 MFMA GEMM
 // of inner dimension 2 NB (rocBLAS), one read and one write.
 #include <algorithm>
@@ -148,14 +151,10 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
   __shared__ double s_scal;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k;
-  if (wave == 0) {
-    double s = sum_partials(a.cnorm, a.ncn);
-    if (lane == 0) s_scal = s;
-  }
-  __syncthreads();
-  Refl R;
-  R.make(s_scal, a.cvec[k]);
 
+  // The product is taken with the raw column c (rows k..m-1), not with the
+  // reflector u = (c + s|c| e_k) / rl: no load waits for the norm of c, and
+  // left_fin applies  A0^T u = (A0^T c + s|c| A0[k, :]) / rl.
   // rows of this thread: r0 + 512 q + {0, 1}, q < NQ (coalesced pairs)
   const int r0 = rb0 + blockIdx.y * (512 * NQ) + 2 * tid;
   double cv[2 * NQ];
@@ -169,14 +168,6 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
   }
 
   if ((int)blockIdx.x < ncb) {
-    double u[2 * NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int r = r0 + 512 * q + h;
-        u[2 * q + h] = (r >= k && r < a.m) ? R.vec(cv[2 * q + h], r == k) : 0.0;
-      }
     const int j0 = k + 1 + blockIdx.x * CBL;
     const int jn = min(CBL, a.n - j0);
     double acc[CBL];
@@ -190,7 +181,7 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
           const int r = r0 + 512 * q;
           if (r < a.lda) {
             const double2 v = ld2(col + r);
-            acc[c] += v.x * u[2 * q] + v.y * u[2 * q + 1];
+            acc[c] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
           }
         }
       }
@@ -204,14 +195,22 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
     if (tid < jn)
       a.Lpart[(int64_t)blockIdx.y * a.ldy + j0 + tid] =
           red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    // the reflector itself goes to its final place, column k
+    // the reflector itself goes to its final place, column k (not read by
+    // any work-group of this launch)
     if (blockIdx.x == 0) {
+      if (wave == 0) {
+        double s = sum_partials(a.cnorm, a.ncn);
+        if (lane == 0) s_scal = s;
+      }
+      __syncthreads();
+      Refl R;
+      R.make(s_scal, a.cvec[k]);
 #pragma unroll
       for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = r0 + 512 * q + h;
-          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = (TS)u[2 * q + h];
+          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = (TS)R.vec(cv[2 * q + h], r == k);
         }
       if (blockIdx.y == 0 && tid == 0) a.alpha[k] = R.diag();
     }
@@ -297,11 +296,12 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
   const double w1 = (td < t) ? (double)a.A[k + (int64_t)(a.p + td) * a.lda]
                              : a.X[k + (int64_t)(td - t) * a.lda];
   const double ds = sum_strided(a.Ldot + td, nchL, 2 * NB);
-  double y = sum_strided(a.Lpart + jc, nchL, a.ldy);
+  const double g = sum_strided(a.Lpart + jc, nchL, a.ldy);  // A0[k:m, j]^T c
 
   Refl R;
   R.make(wave_sum(cn), ck);
   if (dotl) dd[tid] = R.dot(ds, w1);
+  double y = R.dot(g, r);  // A0[k:m, j]^T u, with r = A0[k, j]
   if (tid < t) {
     uk[tid] = (double)a.A[k + (int64_t)(a.p + tid) * a.lda];
     xk[tid] = a.X[k + (int64_t)tid * a.lda];
@@ -337,16 +337,11 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   __shared__ double s_scal;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k;
-  if (wave == 0) {
-    double s = sum_partials(a.rnorm, a.nrn);
-    if (lane == 0) s_scal = s;
-  }
-  __syncthreads();
-  Refl R;
-  R.make(s_scal, a.rvec[k + 1]);
+  // The product is taken with the raw row r (columns k+1..n-1); right_fin
+  // applies  A0 v = (A0 r + s|r| A0[:, k+1]) / rl.
   const int j0 = k + 1 + blockIdx.y * CC;
   const int jn = min(CC, a.n - j0);
-  if (tid < jn) vs[tid] = R.vec(a.rvec[j0 + tid], j0 + tid == k + 1);
+  if (tid < jn) vs[tid] = a.rvec[j0 + tid];
   __syncthreads();
 
   if ((int)blockIdx.x < nrb) {
@@ -375,10 +370,19 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
       if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
     }
     if (blockIdx.x == 0) {
-      __syncthreads();  // row k of this chunk has been read by the loop above
+      // the reflector goes to row k (read by the loop above, hence the
+      // barrier) and to V
+      if (wave == 0) {
+        double s = sum_partials(a.rnorm, a.nrn);
+        if (lane == 0) s_scal = s;
+      }
+      __syncthreads();
+      Refl R;
+      R.make(s_scal, a.rvec[k + 1]);
       if (tid < jn) {
-        a.A[k + (int64_t)(j0 + tid) * a.lda] = (TS)vs[tid];
-        a.V[(j0 + tid) + (int64_t)a.t * a.ldy] = vs[tid];
+        const double v = R.vec(vs[tid], j0 + tid == k + 1);
+        a.A[k + (int64_t)(j0 + tid) * a.lda] = (TS)v;
+        a.V[(j0 + tid) + (int64_t)a.t * a.ldy] = v;
       }
       if (blockIdx.y == 0 && tid == 0) a.beta[k] = R.diag();
     }
@@ -452,12 +456,13 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
                               : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
   const double ds = noright ? 0.0 : sum_strided(a.Rdot + td, nccR, DSTRIDE);
-  double x = noright ? 0.0 : sum_strided(a.Rpart + rc, nccR, a.lda);
+  const double h = noright ? 0.0 : sum_strided(a.Rpart + rc, nccR, a.lda);  // A0[row, k+1:n] r
 
   Refl R;
   R.nrm = R.sgn = R.rl = 0.0;
   if (!noright) R.make(wave_sum(rn), rk1);
   if (dotr) dyv[tid] = R.dot(ds, w1);
+  double x = R.dot(h, c);  // A0[row, k+1:n] v, with c = A0[row, k+1]; 0 if noright
   if (tid <= t) yk1[tid] = a.Y[(k + 1) + (int64_t)tid * a.ldy];
   if (tid < t) vk1[tid] = a.V[(k + 1) + (int64_t)tid * a.ldy];
   if (tid == t) vk1[t] = noright ? 0.0 : R.vec(rk1, true);
