@@ -210,49 +210,6 @@ __host__ __device__ constexpr int pair_q(int i) {
   return KIND == 0 ? B + i : (i / (B / 2)) * B + B - 1 - i % (B / 2);
 }
 
-template <int B>
-struct Log2;
-template <>
-struct Log2<2> {
-  static constexpr int v = 1;
-};
-template <>
-struct Log2<4> {
-  static constexpr int v = 2;
-};
-template <>
-struct Log2<8> {
-  static constexpr int v = 3;
-};
-
-// Wave sum of each g[i], delivered to the lanes of group i (the 64/B lanes
-// with lane >> (6 - log2 B) == i).  The first log2 B butterfly steps halve the
-// set of values a lane carries (it keeps the half its lane bit selects and
-// sends the other half), the remaining steps finish the sum inside the
-// group: (B - 1) + (6 - log2 B) exchanges instead of 6 B.  Every lane of a
-// group ends with the same value.
-template <typename T, int B>
-__device__ __forceinline__ T reduce_scatter(T (&g)[B]) {
-  constexpr int LB = Log2<B>::v;
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int L = 0; L < LB; ++L) {
-    const int bit = 5 - L;
-    const int half = B >> (L + 1);
-    const bool upper = (lane >> bit) & 1;
-#pragma unroll
-    for (int j = 0; j < half; ++j) {
-      const T send = upper ? g[j] : g[j + half];
-      const T keep = upper ? g[j + half] : g[j];
-      g[j] = keep + __shfl_xor(send, 1 << bit, 64);
-    }
-  }
-  T v = g[0];
-#pragma unroll
-  for (int bit = 5 - LB; bit >= 0; --bit) v += __shfl_xor(v, 1 << bit, 64);
-  return v;
-}
-
 // One round: B disjoint rotations.  nrm: lane c holds the squared norm of
 // the column at position c.  The dot products are reduce-scattered across
 // the wave so that the lanes of group i hold pair i's; each group computes

@@ -8,12 +8,14 @@
 // matrix is kept in its panel-start state A0 and every step uses the
 // low-rank form  M_t = A0 - U Y^T - X V^T  of the partially reduced matrix
 // (u_s / v_s: left / right reflectors of the panel, y_s = 2 M^T u_s,
-// x_s = 2 M v_s).  A step t (global column k) is four stream-ordered kernels:
+// x_s = 2 M v_s).  A step t (global column k) is three stream-ordered
+// kernels:
 //
 //   left_gemv   g = A0[k:m, k+1:n]^T c           2-D grid, partial sums
-//   left_fin    A0^T u = (g + s|c| A0[k, :]) / rl;
-//               y_t = 2(A0^T u - Y (U^T u) - V (X^T u)); row k of M -> r
-//   right_gemv  h = A0[k+1:m, k+1:n] r           2-D grid, partial sums
+//   right_gemv  left finalization on each column chunk:
+//                 A0^T u = (g + s|c| A0[k, :]) / rl,
+//                 y_t = 2(A0^T u - Y (U^T u) - V (X^T u)); row k of M -> r;
+//               then h = A0[k+1:m, k+1:n] r      2-D grid, partial sums
 //   right_fin   A0 v = (h + s|r| A0[:, k+1]) / rl;
 //               x_t = 2(A0 v - U (Y^T v) - X (V^T v)); column k+1 of M -> c
 //
@@ -92,16 +94,17 @@ __device__ __forceinline__ double sum_partials(const double* part, int cnt) {
 }
 
 // sum_{q<cnt} p[q*stride] in a fixed order, with the loads issued in
-// batches of 32 (clamped addresses, unused past cnt) so that their
-// latencies overlap.
+// batches of BATCH (clamped addresses, unused past cnt) so that their
+// latencies overlap.  The order of the additions does not depend on BATCH.
+template <int BATCH = 32>
 __device__ __forceinline__ double sum_strided(const double* p, int cnt, int64_t stride) {
   double acc = 0.0;
-  for (int q = 0; q < cnt; q += 32) {
-    double v[32];
+  for (int q = 0; q < cnt; q += BATCH) {
+    double v[BATCH];
 #pragma unroll
-    for (int u = 0; u < 32; ++u) v[u] = p[(int64_t)min(q + u, cnt - 1) * stride];
+    for (int u = 0; u < BATCH; ++u) v[u] = p[(int64_t)min(q + u, cnt - 1) * stride];
 #pragma unroll
-    for (int u = 0; u < 32; ++u)
+    for (int u = 0; u < BATCH; ++u)
       if (q + u < cnt) acc += v[u];
   }
   return acc;
@@ -328,20 +331,113 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
   if (tid == 0) a.rnorm[blockIdx.x] = s;
 }
 
-// right_gemv: grid.x = row blocks + dot blocks, grid.y = column chunks.
-// CC: trailing columns per work-group.
+// right_gemv with the left finalization fused in.  grid.x = nrb row blocks
+// + 1 owner block, grid.y = column chunks of CC = 128 columns.
+//
+// Every work-group first forms row k of M on its chunk, as bd_left_fin does
+// for all columns (bitwise the same values in every work-group: same code,
+// same order).  With P = Y (U^T c) + V (X^T c) and Q = Y U[k, :]^T +
+// V X[k, :]^T (panel terms, j-th entries) the left finalization is
+//   y_t = 2 ((g - P) + s|c| (A0[k, :] - Q)) / rl,   r = A0[k, :] - Q - u_t[k] y_t,
+// so the column threads need the reflector data only at the very end.  The
+// row blocks then stream A0[rows, chunk] r; the owner block stores y_t and r
+// and leaves the chunk's partial sums of |r|^2, y_s^T r (s <= t) and
+// v_s^T r (s < t) for right_fin.  This saves the launch of bd_left_fin.
 template <typename TS, int CC>
-__global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0) {
-  __shared__ double vs[CC];
-  __shared__ double red[4][DPW];
-  __shared__ double s_scal;
+__global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0, int nchL) {
+  static_assert(CC == 128, "two threads per column of the chunk");
+  constexpr int H = NB / 2;             // panel columns per half
+  __shared__ double rs[CC];             // r on the chunk
+  __shared__ double yv[CC];             // y_t on the chunk
+  __shared__ double pq[2][CC];          // second half's P, Q
+  __shared__ double ds[2 * NB];         // u_s^T c, x_s^T c (s < t)
+  __shared__ double uk[NB + 1];         // U[k, s], s <= t (u_t[k] from the reflector)
+  __shared__ double xk[NB];             // X[k, s], s < t
+  __shared__ double rfl[3];             // the left reflector: |c|, sign, rl
+  __shared__ double red[4][32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int k = a.k;
-  // The product is taken with the raw row r (columns k+1..n-1); right_fin
-  // applies  A0 v = (A0 r + s|r| A0[:, k+1]) / rl.
+  const int k = a.k, t = a.t;
   const int j0 = k + 1 + blockIdx.y * CC;
   const int jn = min(CC, a.n - j0);
-  if (tid < jn) vs[tid] = a.rvec[j0 + tid];
+  const int col = tid & (CC - 1), half = tid >> 7;
+  const bool jv = col < jn;
+  const int jc = jv ? j0 + col : k + 1;  // a valid column for the loads of idle lanes
+
+  // Every load first.  Thread (col, half) holds Y[j, s], V[j, s] for the
+  // panel columns s = half H + i < t.
+  double ys[H] = {}, vs[H] = {};
+  const double* Yj = a.Y + jc + (int64_t)(half * H) * a.ldy;
+  const double* Vj = a.V + jc + (int64_t)(half * H) * a.ldy;
+  const int th = t - half * H;  // panel columns of this half
+#pragma unroll
+  for (int g = 0; g < H / 8; ++g) {
+    if (g * 8 < th) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = min(g * 8 + u, th - 1);  // clamped: in bounds, unused past th
+        ys[g * 8 + u] = Yj[(int64_t)i * a.ldy];
+        vs[g * 8 + u] = Vj[(int64_t)i * a.ldy];
+      }
+    }
+  }
+  double gsum = 0.0, rk = 0.0;
+  if (half == 0) {
+    rk = (double)a.A[k + (int64_t)jc * a.lda];
+    gsum = sum_strided<8>(a.Lpart + jc, nchL, a.ldy);  // A0[k:m, j]^T c
+  }
+  if (wave == 2) {
+    // the left reflector (from the partial sums of |c|^2) and u_t[k]
+    double cn = 0.0;
+    for (int i = lane; i < a.ncn; i += 64) cn += a.cnorm[i];
+    const double ck = a.cvec[k];
+    Refl R;
+    R.make(wave_sum(cn), ck);
+    if (lane == 0) {
+      rfl[0] = R.nrm;
+      rfl[1] = R.sgn;
+      rfl[2] = R.rl;
+      uk[t] = R.vec(ck, true);
+    }
+  } else if (wave == 3) {
+    const bool dl = lane < 2 * t;
+    const double d = sum_strided<8>(a.Ldot + (dl ? lane : 0), nchL, 2 * NB);
+    if (dl) ds[lane] = d;
+    if (lane < t) {
+      uk[lane] = (double)a.A[k + (int64_t)(a.p + lane) * a.lda];
+      xk[lane] = a.X[k + (int64_t)lane * a.lda];
+    }
+  }
+  __syncthreads();
+
+  // panel terms of this half
+  double P = 0.0, Q = 0.0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    if (i < th) {
+      const int sx = half * H + i;
+      P += ys[i] * ds[sx] + vs[i] * ds[t + sx];
+      Q += ys[i] * uk[sx] + vs[i] * xk[sx];
+    }
+  }
+  if (half == 1) {
+    pq[0][col] = P;
+    pq[1][col] = Q;
+  }
+  __syncthreads();
+  double y = 0.0, r = 0.0;
+  if (half == 0) {
+    Refl R;
+    R.nrm = rfl[0];
+    R.sgn = rfl[1];
+    R.rl = rfl[2];
+    P += pq[0][col];
+    Q += pq[1][col];
+    y = 2.0 * R.dot(gsum - P, rk - Q);
+    r = rk - Q - uk[t] * y;
+    if (!jv) y = r = 0.0;
+    rs[col] = r;
+    yv[col] = y;
+  }
   __syncthreads();
 
   if ((int)blockIdx.x < nrb) {
@@ -356,68 +452,61 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
         for (int u = 0; u < 8; ++u) v[u] = ld2(base + (int64_t)(q + u) * a.lda);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-          h0 += v[u].x * vs[q + u];
-          h1 += v[u].y * vs[q + u];
+          h0 += v[u].x * rs[q + u];
+          h1 += v[u].y * rs[q + u];
         }
       }
       for (; q < jn; ++q) {
         const double2 v = ld2(base + (int64_t)q * a.lda);
-        h0 += v.x * vs[q];
-        h1 += v.y * vs[q];
+        h0 += v.x * rs[q];
+        h1 += v.y * rs[q];
       }
       double* out = a.Rpart + (int64_t)blockIdx.y * a.lda;
       if (r0 >= k + 1 && r0 < a.m) out[r0] = h0;
       if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
     }
-    if (blockIdx.x == 0) {
-      // the reflector goes to row k (read by the loop above, hence the
-      // barrier) and to V
-      if (wave == 0) {
-        double s = sum_partials(a.rnorm, a.nrn);
-        if (lane == 0) s_scal = s;
-      }
-      __syncthreads();
-      Refl R;
-      R.make(s_scal, a.rvec[k + 1]);
-      if (tid < jn) {
-        const double v = R.vec(vs[tid], j0 + tid == k + 1);
-        a.A[k + (int64_t)(j0 + tid) * a.lda] = (TS)v;
-        a.V[(j0 + tid) + (int64_t)a.t * a.ldy] = v;
-      }
-      if (blockIdx.y == 0 && tid == 0) a.beta[k] = R.diag();
-    }
   } else {
-    // panel dots with the raw row r: y_s^T r (s <= t), v_s^T r (s < t)
-    const int q0 = (blockIdx.x - nrb) * DPW;
-    const int nd = 2 * a.t + 1;
-    const int j = j0 + tid;
-    double r = (tid < jn) ? a.rvec[j] : 0.0;
-    double acc[DPW];
+    // owner: y_t and r, then the chunk's partial sums for right_fin in the
+    // layout of Rdot: [s] y_s^T r (s <= t), [t + 1 + s] v_s^T r (s < t),
+    // and |r|^2 in rnorm.  Slots of the reduction: 0..31 y_s^T r (slot t:
+    // y_t^T r), 32..62 v_s^T r, 63 |r|^2; this half holds slots
+    // half H + i and 32 + half H + i, i < H.
+    if (half == 0 && jv) {
+      a.Y[j0 + col + (int64_t)t * a.ldy] = y;
+      a.rvec[j0 + col] = r;
+    }
+    const double rr = rs[col], yy = yv[col];
+    double g[2 * H];
 #pragma unroll
-    for (int q = 0; q < DPW; ++q) {
-      acc[q] = 0.0;
-      int s = q0 + q;
-      if (s < nd && tid < jn) {
-        double w = (s <= a.t) ? a.Y[j + (int64_t)s * a.ldy]
-                              : a.V[j + (int64_t)(s - a.t - 1) * a.ldy];
-        acc[q] = w * r;
+    for (int i = 0; i < H; ++i) {
+      const int sx = half * H + i;
+      g[i] = (i < th) ? ys[i] * rr : (sx == t ? yy * rr : 0.0);
+      g[H + i] = (i < th) ? vs[i] * rr : (sx == NB - 1 ? rr * rr : 0.0);
+    }
+    const double tot = reduce_scatter<double, 2 * H>(g);  // lane l: slot index l >> 1
+    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
+    __syncthreads();
+    if (tid < 64) {
+      const int hf = tid >> 5, i = tid & 31;  // half, index within the half's 2H slots
+      const double v = red[2 * hf][i] + red[2 * hf + 1][i];
+      const bool ysl = i < H;
+      const int sx = hf * H + (ysl ? i : i - H);
+      double* Rd = a.Rdot + (int64_t)blockIdx.y * DSTRIDE;
+      if (ysl) {
+        if (sx <= t) Rd[sx] = v;
+      } else if (sx < t) {
+        Rd[t + 1 + sx] = v;
+      } else if (sx == NB - 1) {
+        a.rnorm[blockIdx.y] = v;
       }
     }
-#pragma unroll
-    for (int q = 0; q < DPW; ++q) {
-      double s = wave_sum(acc[q]);
-      if (lane == 0) red[wave][q] = s;
-    }
-    __syncthreads();
-    if (tid < DPW && q0 + tid < nd)
-      a.Rdot[(int64_t)blockIdx.y * DSTRIDE + q0 + tid] =
-          red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
   }
 }
 
-// right_fin: one thread per row in [k+1, m), one wave per group.  noright:
-// the row of M has a single entry, taken as beta with no reflection
-// (bidiag.c:125-129).
+// right_fin: one thread per row in [k+1, m), one wave per group; thread i
+// of the grid also stores entry k+1+i of the right reflector (m >= n, so
+// the grid covers the row).  noright: the row of M has a single entry,
+// taken as beta with no reflection (bidiag.c:125-129).
 template <typename TS>
 __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright) {
   __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
@@ -451,6 +540,9 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   if (!noright)
     for (int i = tid; i < a.nrn; i += FIN) rn += a.rnorm[i];
   const double rk1 = a.rvec[k + 1];
+  // this thread's column of the right reflector (written below)
+  const int jw = k + 1 + blockIdx.x * FIN + tid;
+  const double rw = a.rvec[jw < a.n ? jw : k + 1];
   const bool dotr = !noright && tid < 2 * t + 1;
   const int td = dotr ? tid : 0;
   const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
@@ -490,6 +582,16 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   }
   double s = wave_sum(cc2);
   if (tid == 0) a.cnorm[blockIdx.x] = s;
+  // the right reflector to row k of A and to column t of V (no work-group
+  // of this launch reads either: rows > k of A, V[k+1, s < t])
+  if (!noright) {
+    if (jw < a.n) {
+      const double v = R.vec(rw, jw == k + 1);
+      a.A[k + (int64_t)jw * a.lda] = (TS)v;
+      a.V[jw + (int64_t)t * a.ldy] = v;
+    }
+    if (blockIdx.x == 0 && tid == 0) a.beta[k] = R.diag();
+  }
   if (noright && blockIdx.x == 0 && tid == 0) {
     a.beta[k] = a.rvec[k + 1];
     a.A[k + (int64_t)(k + 1) * a.lda] = (TS)0;
@@ -685,25 +787,26 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
                        ncb, rb0);
     DDC_LAUNCHED();
     if (kl0 >= 0) kclock_span(KCLOCK_LEFT_GEMV, kl0, kclock_mark(st));
-    const int nfl = (int)ceil_div(n - k - 1, FIN);
-    hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);
-    DDC_LAUNCHED();
-    a.nrn = nfl;
-    // right reflector of row k and the right GEMV
+    // row k of M (the left finalization), the right reflector of row k and
+    // the right GEMV
     const int nfr = (int)ceil_div(m - k - 1, FIN);
     if (k == n - 2) {
+      const int nfl = (int)ceil_div(n - k - 1, FIN);
+      hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);
+      DDC_LAUNCHED();
+      a.nrn = nfl;
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
       DDC_LAUNCHED();
     } else {
       const int rbr = ((k + 1) / 64) * 64;
       const int nrb = (int)ceil_div(m - rbr, RBR);
-      const int ndr = (int)ceil_div(2 * a.t + 1, DPW);
       const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int kr0 = kclock_on() ? kclock_mark(st) : -1;
-      hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + ndr, ncc), dim3(256), 0, st, a,
-                         nrb, rbr);
+      hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st, a,
+                         nrb, rbr, nchL);
       DDC_LAUNCHED();
       if (kr0 >= 0) kclock_span(KCLOCK_RIGHT_GEMV, kr0, kclock_mark(st));
+      a.nrn = ncc;
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();
     }

@@ -169,4 +169,60 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+template <int B>
+struct Log2;
+template <>
+struct Log2<2> {
+  static constexpr int v = 1;
+};
+template <>
+struct Log2<4> {
+  static constexpr int v = 2;
+};
+template <>
+struct Log2<8> {
+  static constexpr int v = 3;
+};
+template <>
+struct Log2<16> {
+  static constexpr int v = 4;
+};
+template <>
+struct Log2<32> {
+  static constexpr int v = 5;
+};
+
+// Wave sum of each g[i], delivered to the lanes of group i (the 64/B lanes
+// with lane >> (6 - log2 B) == i).  The first log2 B butterfly steps halve the
+// set of values a lane carries (it keeps the half its lane bit selects and
+// sends the other half), the remaining steps finish the sum inside the
+// group: (B - 1) + (6 - log2 B) exchanges instead of 6 B.  Every lane of a
+// group ends with the same value.
+template <typename T, int B>
+__device__ __forceinline__ T reduce_scatter(T (&g)[B]) {
+  constexpr int LB = Log2<B>::v;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int L = 0; L < LB; ++L) {
+    const int bit = 5 - L;
+    const int half = B >> (L + 1);
+    const bool upper = (lane >> bit) & 1;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      // the empty asm keeps both values in registers: without it the two
+      // selects become one load from a selected address, which sends the
+      // whole array to scratch memory
+      T lo = g[j], hi = g[j + half];
+      asm volatile("" : "+v"(lo), "+v"(hi));
+      const T send = upper ? lo : hi;
+      const T keep = upper ? hi : lo;
+      g[j] = keep + __shfl_xor(send, 1 << bit, 64);
+    }
+  }
+  T v = g[0];
+#pragma unroll
+  for (int bit = 5 - LB; bit >= 0; --bit) v += __shfl_xor(v, 1 << bit, 64);
+  return v;
+}
+
 }  // namespace ddc
