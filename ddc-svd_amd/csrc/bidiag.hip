@@ -57,6 +57,7 @@ struct StepArgs {
   double* Y;  // n x NB, ld ldy: y_s
   double* V;  // n x NB, ld ldy: v_s (contiguous copy of the row reflectors)
   double* X;  // m x NB, ld lda: x_s
+  double* Ucopy;  // m x NB, ld lda: FP64 copy of u_s for the trailing update, or null
   double* cvec;  // current column of M (rows k..m-1)
   double* rvec;  // current row of M (columns k+1..n-1)
   double* Lpart; // left_gemv partials [chunk][ldy]
@@ -213,7 +214,11 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int r = r0 + 512 * q + h;
-          if (r >= k && r < a.m) a.A[r + (int64_t)k * a.lda] = (TS)R.vec(cv[2 * q + h], r == k);
+          if (r >= k && r < a.m) {
+            const double u = R.vec(cv[2 * q + h], r == k);
+            a.A[r + (int64_t)k * a.lda] = (TS)u;
+            if (a.Ucopy) a.Ucopy[r + (int64_t)a.t * a.lda] = u;
+          }
         }
       if (blockIdx.y == 0 && tid == 0) a.alpha[k] = R.diag();
     }
@@ -730,10 +735,12 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
   a.beta = beta;
   a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
   a.V = a.Y + (size_t)d.ldy * NB;
-  // [U X]: the panel's left reflectors are copied next to X before the
-  // trailing update, so that [U X] and [Y V] are single dense operands
+  // [U X] and [Y V]: the trailing update's two dense operands
   double* UX = ws.take<double>((size_t)lda * NB * 2);
   a.X = UX ? UX + (size_t)lda * NB : nullptr;
+  // FP64 storage: left_gemv also writes each reflector next to X, so that
+  // [U X] is one dense operand of the trailing update
+  a.Ucopy = sizeof(TS) == 8 ? UX : nullptr;
   a.cvec = ws.take<double>(lda);
   a.rvec = ws.take<double>(d.ldy);
   a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
@@ -816,9 +823,6 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       const int r0 = a.p + NB;
       TS* C = A + r0 + (int64_t)r0 * lda;
       if constexpr (sizeof(TS) == 8) {
-        DDC_HIP(hipMemcpy2DAsync(UX + r0, sizeof(double) * lda, A + r0 + (int64_t)a.p * lda,
-                                 sizeof(double) * lda, sizeof(double) * (m - r0), NB,
-                                 hipMemcpyDeviceToDevice, st));
         DDC_TRY(dgemm('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0, d.ldy,
                       1.0, C, lda, st));
       } else {
