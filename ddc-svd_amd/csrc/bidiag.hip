@@ -190,11 +190,9 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
         }
       }
     }
-#pragma unroll
-    for (int c = 0; c < CBL; ++c) {
-      double s = wave_sum(acc[c]);
-      if (lane == 0) red[wave][c] = s;
-    }
+    // lanes 8c .. 8c+7 end with column c's wave sum
+    const double sc = reduce_scatter<double, CBL>(acc);
+    if ((lane & (64 / CBL - 1)) == 0) red[wave][lane / (64 / CBL)] = sc;
     __syncthreads();
     if (tid < jn)
       a.Lpart[(int64_t)blockIdx.y * a.ldy + j0 + tid] =
@@ -255,11 +253,8 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
         }
       }
     }
-#pragma unroll
-    for (int e = 0; e < DPW; ++e) {
-      double s = wave_sum(acc[e]);
-      if (lane == 0) red[wave][e] = s;
-    }
+    const double se = reduce_scatter<double, DPW>(acc);
+    if ((lane & (64 / DPW - 1)) == 0) red[wave][lane / (64 / DPW)] = se;
     __syncthreads();
     if (tid < DPW && q0 + tid < nd)
       a.Ldot[(int64_t)blockIdx.y * (2 * NB) + q0 + tid] =
