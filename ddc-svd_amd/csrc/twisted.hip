@@ -284,19 +284,39 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
   if (rep == 0) a.sig_out[idx] = sqrt(fmax(lam + corr, 0.0));
 }
 
-// u_i^T B v_i >= 0
-__global__ void tw_sign(int n, const double* a, const double* b, const double* Vt,
-                        double* Ut, int ldv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// u_i^T B v_i >= 0.  Lane = singular value i; the four waves of a group
+// split the index range j and their partial sums meet in LDS.  Loads are
+// issued eight indices at a time so that their latencies overlap.
+__global__ __launch_bounds__(256) void tw_sign(int n, const double* a, const double* b,
+                                               const double* Vt, double* Ut, int ldv) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int ic = i < n ? i : n - 1;  // a valid column for the loads of idle lanes
+  const int q = (n + 3) / 4;
+  const int j0 = wave * q, j1 = min(n, j0 + q);
   double t = 0.0;
-  for (int j = 0; j < n; ++j) {
-    double bv = a[j] * Vt[(int64_t)j * ldv + i];
-    if (j < n - 1) bv += b[j] * Vt[(int64_t)(j + 1) * ldv + i];
-    t += Ut[(int64_t)j * ldv + i] * bv;
+  for (int j = j0; j < j1; j += 8) {
+    double av[8], bv[8], vj[8], vn[8], uj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int jj = min(j + u, n - 1);         // clamped: in bounds, unused past j1
+      const int jn = min(jj + 1, n - 1);
+      av[u] = a[jj];
+      bv[u] = jj < n - 1 ? b[jj] : 0.0;
+      vj[u] = Vt[(int64_t)jj * ldv + ic];
+      vn[u] = Vt[(int64_t)jn * ldv + ic];
+      uj[u] = Ut[(int64_t)jj * ldv + ic];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (j + u < j1) t += uj[u] * (av[u] * vj[u] + bv[u] * vn[u]);
   }
-  if (t < 0.0)
-    for (int j = 0; j < n; ++j) Ut[(int64_t)j * ldv + i] = -Ut[(int64_t)j * ldv + i];
+  part[wave][lane] = t;
+  __syncthreads();
+  const double tot = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+  if (i < n && tot < 0.0)
+    for (int j = j0; j < j1; ++j) Ut[(int64_t)j * ldv + i] = -Ut[(int64_t)j * ldv + i];
 }
 
 static size_t tw_ck_size(int n) { return (size_t)((n + TW_CK - 1) / TW_CK) * 2 * n; }
@@ -337,7 +357,8 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   t.smax = smax;
   hipLaunchKernelGGL(tw_solve, dim3(2 * (unsigned)ceil_div(n, 64)), dim3(64), 0, st, t);
   DDC_LAUNCHED();
-  hipLaunchKernelGGL(tw_sign, dim3((unsigned)ceil_div(n, 64)), dim3(64), 0, st, n, a, b, Vt, Ut, ldv);
+  hipLaunchKernelGGL(tw_sign, dim3((unsigned)ceil_div(n, 64)), dim3(256), 0, st, n, a, b, Vt,
+                     Ut, ldv);
   DDC_LAUNCHED();
   return DDC_OK;
 }
