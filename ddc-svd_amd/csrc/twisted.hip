@@ -72,28 +72,43 @@ __device__ __forceinline__ double guard(double x, double pivmin) {
   return (fabs(x) < pivmin) ? -pivmin : x;
 }
 
-// One wave per group, all lanes on the same representation; lane = one
-// singular value.  The lanes walk the index range in lockstep, chunk by
-// chunk: the chunk's qd entries are staged in LDS, and the recurrences the
-// twisted factorization needs in the opposite direction are recomputed for
-// the chunk from checkpoints (every TW_CK steps) into a per-lane LDS column.
-// Nothing of size n per lane is stored except the vector itself.
+// Two waves per group of 64 singular values, all lanes on the same
+// representation; lane = one singular value, and lane l of both waves works
+// on the same value (their per-value state is kept identical).  The lanes
+// walk the index range in lockstep, chunk by chunk; the chunk's qd entries
+// are staged in LDS, and the recurrences that a pass needs in the opposite
+// direction are recomputed for the chunk from checkpoints (every TW_CK
+// steps) into a per-wave LDS buffer.  Nothing of size n per lane is stored
+// except the vector itself.  The two waves run the independent recurrences
+// of each pass side by side, on different SIMDs:
+//   pass 1   wave 0: stationary transform, ascending (checkpoints of s,
+//            Sturm count at lam); wave 1: progressive transform,
+//            descending (checkpoints of p)
+//   pass 2   per chunk, wave 0 recomputes s and wave 1 recomputes p; gamma
+//            and its argmin over the chunk's two halves
+//   pass 3   wave 0: z downward from the twist (recomputed p); wave 1: z
+//            upward (recomputed s)
 //
 // Recurrences (L D L^T - lam I, d = a^2, ld = a b, lld = b^2):
 //   stationary  s_0 = -lam, D+_i = d_i + s_i, s_{i+1} = lld_i s_i / D+_i - lam
 //   progressive p_{n-1} = d_{n-1} - lam, D-_{i+1} = lld_i + p_{i+1},
 //               p_i = d_i p_{i+1} / D-_{i+1} - lam
-//   twist       gamma_i = s_i + p_i + lam, r = argmin |gamma_i|
+//   twist       gamma_i = s_i + p_i + lam, r = argmin |gamma_i| (the largest
+//               such i on ties)
 //   vector      z_r = 1, z_{i+1} = -(ld_i / D-_{i+1}) z_i  (i >= r),
 //               z_i = -(ld_i / D+_i) z_{i+1}  (i < r)
 //   correction  lam += gamma_r / |z|^2   (Rayleigh quotient)
-__global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
-  __shared__ double qd_d[TW_CK], qd_ld[TW_CK], qd_lld[TW_CK];
-  __shared__ double buf[TW_CK][64];
+__global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
+  // staged qd of the chunk each wave works on: [wave][d, ld, lld][e]
+  __shared__ double qs[2][3][TW_CK];
+  __shared__ double buf[2][TW_CK][64];  // per-wave recomputed s or p, [e][lane]
+  // exchange between the waves, [wave][lane]
+  __shared__ double xg[2][64], xn[2][64];
+  __shared__ int xr[2][64], xc[64];
   const int n = a.n;
   const int nbr = (n + 63) / 64;
   const int rep = ((int)blockIdx.x >= nbr) ? 1 : 0;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int idx = ((int)blockIdx.x - rep * nbr) * 64 + lane;
   const bool live = idx < n;
   const double* d = a.qd + (size_t)rep * 3 * n;
@@ -107,15 +122,21 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
   const int64_t ldv = a.ldv;
   auto oidx = [&](int j) -> int64_t { return (int64_t)(rep ? (n - 1 - j) : j) * ldv; };
   const int nck = (n + TW_CK - 1) / TW_CK;
-  auto stage = [&](int c0, int len) {
+  // both waves stage one chunk each (cw: this wave's chunk)
+  auto stage = [&](int cw) {
     __syncthreads();
+    const int c0 = cw * TW_CK, len = min(TW_CK, n - c0);
     if (lane < len) {
-      qd_d[lane] = d[c0 + lane];
-      qd_ld[lane] = ld[c0 + lane];
-      qd_lld[lane] = lld[c0 + lane];
+      qs[wave][0][lane] = d[c0 + lane];
+      qs[wave][1][lane] = ld[c0 + lane];
+      qs[wave][2][lane] = lld[c0 + lane];
     }
     __syncthreads();
   };
+  const double* qd_d = qs[wave][0];
+  const double* qd_ld = qs[wave][1];
+  const double* qd_lld = qs[wave][2];
+  double (*bw)[64] = buf[wave];
 
   const double smax = a.smax;
   const double pivmin = TW_TINY * fmax(1.0, smax * smax);
@@ -126,22 +147,26 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
   double hi = (sig + dl) * (sig + dl);
   const double dn1 = d[n - 1];
 
-  // Sturm counts at lo and hi (two independent chains)
+  // Sturm counts: wave 0 at lo, wave 1 at hi
   {
-    double s1 = -lo, s2 = -hi;
-    int c1 = 0, c2 = 0;
+    const double sh = wave == 0 ? lo : hi;
+    double s = -sh;
+    int cn = 0;
     for (int c = 0; c < nck; ++c) {
-      const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
-      stage(c0, len);
-      for (int e = 0; e < len; ++e) {
-        double p1 = guard(qd_d[e] + s1, pivmin);
-        double p2 = guard(qd_d[e] + s2, pivmin);
-        c1 += (p1 < 0.0);
-        c2 += (p2 < 0.0);
-        s1 = qd_lld[e] * s1 / p1 - lo;
-        s2 = qd_lld[e] * s2 / p2 - hi;
+      stage(c);
+      const int len = min(TW_CK, n - c * TW_CK);
+#pragma unroll 8
+      for (int e = 0; e < TW_CK; ++e) {
+        if (e < len) {
+          const double p = guard(qd_d[e] + s, pivmin);
+          cn += (p < 0.0);
+          s = qd_lld[e] * s / p - sh;
+        }
       }
     }
+    xr[wave][lane] = cn;
+    __syncthreads();
+    const int c1 = xr[0][lane], c2 = xr[1][lane];
     if (c1 > idx) lo = 0.0;
     if (c2 < idx + 1) hi = (smax + dl) * (smax + dl) * (1.0 + 1e-8);
   }
@@ -150,109 +175,169 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
   double nz2 = 1.0, corr = 0.0, prev = INFINITY;
   bool active = live;
   for (int it = 0; it < 10; ++it) {
-    if (__ballot(active) == 0) break;
-    // stationary transform with checkpoints and the Sturm count at lam
+    if (__ballot(active) == 0) break;  // the same in both waves
+    // pass 1: stationary (wave 0, ascending) and progressive (wave 1,
+    // descending) transforms with checkpoints
     int cnt = 0;
     {
-      double s = -lam;
+      double s = -lam;           // wave 0
+      double p = dn1 - lam;      // wave 1
       for (int c = 0; c < nck; ++c) {
-        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
-        stage(c0, len);
-        if (active) Sck[(int64_t)c * T] = s;
-        for (int e = 0; e < len; ++e) {
-          double dp = guard(qd_d[e] + s, pivmin);
-          cnt += (dp < 0.0);
-          s = qd_lld[e] * s / dp - lam;  // past the last index: unused
-        }
-      }
-    }
-    // progressive transform (descending) with checkpoints; the chunk's s
-    // values are recomputed into LDS to form gamma and its argmin
-    int r = n - 1;
-    double gbest = INFINITY;
-    {
-      double p = dn1 - lam;
-      for (int c = nck - 1; c >= 0; --c) {
-        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
-        stage(c0, len);
-        double ss = Sck[(int64_t)c * T];
-        for (int e = 0; e < len; ++e) {
-          buf[e][lane] = ss;
-          double dp = guard(qd_d[e] + ss, pivmin);
-          ss = qd_lld[e] * ss / dp - lam;
-        }
-        for (int e = len - 1; e >= 0; --e) {
-          const int i = c0 + e;
-          if (i < n - 1) {
-            double dm = guard(qd_lld[e] + p, pivmin);
-            p = qd_d[e] * p / dm - lam;
+        const int cw = wave == 0 ? c : nck - 1 - c;
+        stage(cw);
+        const int c0 = cw * TW_CK, len = min(TW_CK, n - c0);
+        if (wave == 0) {
+          if (active) Sck[(int64_t)cw * T] = s;
+#pragma unroll 8
+          for (int e = 0; e < TW_CK; ++e) {
+            if (e < len) {
+              const double dp = guard(qd_d[e] + s, pivmin);
+              cnt += (dp < 0.0);
+              s = qd_lld[e] * s / dp - lam;  // past the last index: unused
+            }
           }
-          const double g = buf[e][lane] + p + lam;
-          if (fabs(g) < fabs(gbest)) {
-            gbest = g;
-            r = i;
-          }
-        }
-        if (active) Pck[(int64_t)c * T] = p;  // p at the chunk's first index
-      }
-    }
-    // z downward from r: needs p_{i+1}, recomputed per chunk from the next
-    // chunk's checkpoint
-    if (active) {
-      nz2 = 1.0;  // converged lanes keep the norm of their final vector
-      out[oidx(r)] = 1.0;
-    }
-    {
-      double z = 1.0;
-      for (int c = 0; c < nck; ++c) {
-        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
-        stage(c0, len);
-        double pp;
-        int etop;
-        if (c + 1 < nck) {
-          pp = Pck[(int64_t)(c + 1) * T];  // p_{c0+len}
-          etop = len - 1;
         } else {
-          pp = dn1 - lam;  // p_{n-1}
-          etop = len - 2;
+#pragma unroll 8
+          for (int e = TW_CK - 1; e >= 0; --e) {
+            if (e < len && c0 + e < n - 1) {
+              const double dm = guard(qd_lld[e] + p, pivmin);
+              p = qd_d[e] * p / dm - lam;
+            }
+          }
+          if (active) Pck[(int64_t)cw * T] = p;  // p at the chunk's first index
         }
-        for (int e = etop; e >= 0; --e) {
-          buf[e][lane] = pp;  // p_{c0+e+1}
-          double dm = guard(qd_lld[e] + pp, pivmin);
-          pp = qd_d[e] * pp / dm - lam;
+      }
+    }
+    // pass 2: gamma_i = s_i + p_i + lam over each chunk, both recomputed
+    int r = -1;  // no finite candidate yet
+    double gbest = INFINITY;
+    for (int c = 0; c < nck; ++c) {
+      stage(c);
+      const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
+      if (wave == 0) {
+        double ss = Sck[(int64_t)c * T];
+#pragma unroll 8
+        for (int e = 0; e < TW_CK; ++e) {
+          if (e < len) {
+            bw[e][lane] = ss;
+            const double dp = guard(qd_d[e] + ss, pivmin);
+            ss = qd_lld[e] * ss / dp - lam;
+          }
         }
-        for (int e = 0; e <= etop; ++e) {
-          const int i = c0 + e;
-          if (active && i >= r) {
-            z = -(qd_ld[e] / guard(qd_lld[e] + buf[e][lane], pivmin)) * z;
-            nz2 += z * z;
-            out[oidx(i + 1)] = z;
+      } else {
+        double pp = (c + 1 < nck) ? Pck[(int64_t)(c + 1) * T] : 0.0;  // p_{c0+len}
+#pragma unroll 8
+        for (int e = TW_CK - 1; e >= 0; --e) {
+          if (e < len) {
+            if (c0 + e == n - 1) {
+              pp = dn1 - lam;
+            } else {
+              const double dm = guard(qd_lld[e] + pp, pivmin);
+              pp = qd_d[e] * pp / dm - lam;
+            }
+            bw[e][lane] = pp;
+          }
+        }
+      }
+      __syncthreads();
+      // wave 0 takes e < 32 of the chunk, wave 1 e >= 32
+#pragma unroll 4
+      for (int q = 0; q < TW_CK / 2; ++q) {
+        const int e = wave * (TW_CK / 2) + q;
+        if (e < len) {
+          const double g = buf[0][e][lane] + buf[1][e][lane] + lam;
+          if (fabs(g) < fabs(gbest) || (fabs(g) == fabs(gbest) && fabs(g) < INFINITY)) {
+            gbest = g;
+            r = c0 + e;
           }
         }
       }
     }
-    // z upward from r: needs s_i, recomputed per chunk from its checkpoint
+    // combine: the smaller |gamma|, the larger index on ties; with no
+    // finite gamma at all the twist stays at the last index
+    {
+      xg[wave][lane] = gbest;
+      xr[wave][lane] = r;
+      if (wave == 0) xc[lane] = cnt;
+      __syncthreads();
+      const double g0 = xg[0][lane], g1 = xg[1][lane];
+      const int r0 = xr[0][lane], r1 = xr[1][lane];
+      cnt = xc[lane];
+      const bool take1 =
+          r1 >= 0 && (r0 < 0 || fabs(g1) < fabs(g0) || (fabs(g1) == fabs(g0) && r1 > r0));
+      if (take1) {
+        gbest = g1;
+        r = r1;
+      } else if (r0 >= 0) {
+        gbest = g0;
+        r = r0;
+      } else {
+        gbest = INFINITY;
+        r = n - 1;
+      }
+    }
+    // pass 3: wave 0 z downward from r (needs p_{i+1}), wave 1 upward (s_i)
+    double nzp = wave == 0 ? 1.0 : 0.0;
+    if (active && wave == 0) out[oidx(r)] = 1.0;
     {
       double z = 1.0;
-      for (int c = nck - 1; c >= 0; --c) {
-        const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
-        stage(c0, len);
-        double ss = Sck[(int64_t)c * T];
-        for (int e = 0; e < len; ++e) {
-          buf[e][lane] = ss;
-          double dp = guard(qd_d[e] + ss, pivmin);
-          ss = qd_lld[e] * ss / dp - lam;
-        }
-        for (int e = len - 1; e >= 0; --e) {
-          const int i = c0 + e;
-          if (active && i < r) {
-            z = -(qd_ld[e] / guard(qd_d[e] + buf[e][lane], pivmin)) * z;
-            nz2 += z * z;
-            out[oidx(i)] = z;
+      for (int c = 0; c < nck; ++c) {
+        const int cw = wave == 0 ? c : nck - 1 - c;
+        stage(cw);
+        const int c0 = cw * TW_CK, len = min(TW_CK, n - c0);
+        if (wave == 0) {
+          double pp;
+          int etop;
+          if (cw + 1 < nck) {
+            pp = Pck[(int64_t)(cw + 1) * T];  // p_{c0+len}
+            etop = len - 1;
+          } else {
+            pp = dn1 - lam;  // p_{n-1}
+            etop = len - 2;
+          }
+#pragma unroll 8
+          for (int e = TW_CK - 1; e >= 0; --e) {
+            if (e <= etop) {
+              bw[e][lane] = pp;  // p_{c0+e+1}
+              const double dm = guard(qd_lld[e] + pp, pivmin);
+              pp = qd_d[e] * pp / dm - lam;
+            }
+          }
+#pragma unroll 8
+          for (int e = 0; e < TW_CK; ++e) {
+            const int i = c0 + e;
+            if (e <= etop && active && i >= r) {
+              z = -(qd_ld[e] / guard(qd_lld[e] + bw[e][lane], pivmin)) * z;
+              nzp += z * z;
+              out[oidx(i + 1)] = z;
+            }
+          }
+        } else {
+          double ss = Sck[(int64_t)cw * T];
+#pragma unroll 8
+          for (int e = 0; e < TW_CK; ++e) {
+            if (e < len) {
+              bw[e][lane] = ss;
+              const double dp = guard(qd_d[e] + ss, pivmin);
+              ss = qd_lld[e] * ss / dp - lam;
+            }
+          }
+#pragma unroll 8
+          for (int e = TW_CK - 1; e >= 0; --e) {
+            const int i = c0 + e;
+            if (e < len && active && i < r) {
+              z = -(qd_ld[e] / guard(qd_d[e] + bw[e][lane], pivmin)) * z;
+              nzp += z * z;
+              out[oidx(i)] = z;
+            }
           }
         }
       }
     }
+    xn[wave][lane] = nzp;
+    __syncthreads();
+    // converged lanes keep the norm of their final vector
+    if (active) nz2 = xn[0][lane] + xn[1][lane];
     if (!active) continue;
     corr = gbest / nz2;
     // converged: correction at the rounding level of lam, the vector's norm
@@ -271,17 +356,20 @@ __global__ __launch_bounds__(64) void tw_solve(TwArgs a) {
     if (!(nl > lo && nl < hi)) nl = 0.5 * (lo + hi);
     lam = nl;
   }
+  __syncthreads();  // both waves' vector entries are written
   if (!live) return;
   const double inv = 1.0 / sqrt(nz2);
-  for (int j0 = 0; j0 < n; j0 += 8) {
+  const int jh = (n + 1) / 2;
+  const int ja = wave == 0 ? 0 : jh, jb = wave == 0 ? jh : n;
+  for (int j0 = ja; j0 < jb; j0 += 8) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = out[(int64_t)min(j0 + u, n - 1) * ldv];
+    for (int u = 0; u < 8; ++u) v[u] = out[(int64_t)min(j0 + u, jb - 1) * ldv];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
-      if (j0 + u < n) out[(int64_t)(j0 + u) * ldv] = v[u] * inv;
+      if (j0 + u < jb) out[(int64_t)(j0 + u) * ldv] = v[u] * inv;
   }
-  if (rep == 0) a.sig_out[idx] = sqrt(fmax(lam + corr, 0.0));
+  if (rep == 0 && wave == 0) a.sig_out[idx] = sqrt(fmax(lam + corr, 0.0));
 }
 
 // u_i^T B v_i >= 0.  Lane = singular value i; the four waves of a group
@@ -355,7 +443,7 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   t.Sck = scr;
   t.Pck = scr + tw_ck_size(n);
   t.smax = smax;
-  hipLaunchKernelGGL(tw_solve, dim3(2 * (unsigned)ceil_div(n, 64)), dim3(64), 0, st, t);
+  hipLaunchKernelGGL(tw_solve, dim3(2 * (unsigned)ceil_div(n, 64)), dim3(128), 0, st, t);
   DDC_LAUNCHED();
   hipLaunchKernelGGL(tw_sign, dim3((unsigned)ceil_div(n, 64)), dim3(256), 0, st, n, a, b, Vt,
                      Ut, ldv);
