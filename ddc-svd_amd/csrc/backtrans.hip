@@ -24,8 +24,9 @@ namespace ddc {
 constexpr int NBB = 512;  // reflectors per compact-WY block
 
 // C(0:n, 0:n) = Bt^T where Bt[j*ldb + i] holds vector i, entry j;
-// rows n..rows-1 of C are zeroed.
-__global__ void bt_unpack(int n, int rows, const double* Bt, int ldb, double* C, int ldc) {
+// rows n..rows-1 of C are zeroed.  TC: element type of C.
+template <typename TC>
+__global__ void bt_unpack(int n, int rows, const double* Bt, int ldb, TC* C, int ldc) {
   __shared__ double tile[32][33];
   int bx = blockIdx.x * 32, by = blockIdx.y * 32;  // bx: i (vector), by: j (entry)
   int tx = threadIdx.x, ty = threadIdx.y;          // 32 x 8
@@ -37,7 +38,7 @@ __global__ void bt_unpack(int n, int rows, const double* Bt, int ldb, double* C,
   // write C(j, i): consecutive threads take consecutive j (rows)
   for (int r = ty; r < 32; r += 8) {
     int i = bx + r, j = by + tx;
-    if (i < n && j < rows) C[(int64_t)j + (int64_t)i * ldc] = (j < n) ? tile[tx][r] : 0.0;
+    if (i < n && j < rows) C[(int64_t)j + (int64_t)i * ldc] = (TC)((j < n) ? tile[tx][r] : 0.0);
   }
 }
 
@@ -69,20 +70,20 @@ int transpose_f64(int r, int c, const double* S, int lds, double* D, int ldd, hi
 // outside their support.  left: Wd(i, c) = A[k0+i, k0+c] for i >= c.
 // right: Wd(i, c) = A[k0+c, k0+1+i] for i >= c (rows of A, read through an
 // LDS tile so that both sides stay coalesced).
-template <bool RIGHT>
-__global__ void bt_pack(int rows, int nb, const double* S, int lda, double* Wd, int ldw) {
-  __shared__ double tile[32][33];
+template <bool RIGHT, typename T>
+__global__ void bt_pack(int rows, int nb, const T* S, int lda, T* Wd, int ldw) {
+  __shared__ T tile[32][33];
   const int i0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
   const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
   if (!RIGHT) {
     for (int q = ty; q < 32; q += 8) {
       const int i = i0 + tx, c = c0 + q;
-      if (i < rows && c < nb) Wd[i + (int64_t)c * ldw] = i >= c ? S[i + (int64_t)c * lda] : 0.0;
+      if (i < rows && c < nb) Wd[i + (int64_t)c * ldw] = i >= c ? S[i + (int64_t)c * lda] : T(0);
     }
   } else {
     for (int q = ty; q < 32; q += 8) {  // tile[c][i] = S(c, i), coalesced over c
       const int c = c0 + tx, i = i0 + q;
-      tile[q][tx] = (c < nb && i < rows && i >= c) ? S[c + (int64_t)i * lda] : 0.0;
+      tile[q][tx] = (c < nb && i < rows && i >= c) ? S[c + (int64_t)i * lda] : T(0);
     }
     __syncthreads();
     for (int q = ty; q < 32; q += 8) {
@@ -94,11 +95,28 @@ __global__ void bt_pack(int rows, int nb, const double* S, int lda, double* Wd, 
 
 // T^{-1} = I/2 + strict_upper(G) (nb x nb, ld nb) for the block
 // H_0 ... H_{nb-1} = I - W T W^T of unit-norm reflectors (tau = 2).
-__global__ void bt_tinv(int nb, const double* G, double* Ti) {
+template <typename T>
+__global__ void bt_tinv(int nb, const T* G, T* Ti) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= nb * nb) return;
   const int i = e % nb, j = e / nb;
-  Ti[e] = i < j ? G[e] : (i == j ? 0.5 : 0.0);
+  Ti[e] = i < j ? G[e] : (i == j ? T(0.5) : T(0));
+}
+
+// Library GEMM / triangular solve in the precision of the block.
+static int gemm_t(char ta, char tb, int M, int N, int K, double alpha, const double* A, int lda,
+                  const double* B, int ldb, double beta, double* C, int ldc, hipStream_t st) {
+  return dgemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+}
+static int gemm_t(char ta, char tb, int M, int N, int K, float alpha, const float* A, int lda,
+                  const float* B, int ldb, float beta, float* C, int ldc, hipStream_t st) {
+  return sgemm(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+}
+static int trsm_t(int M, int N, const double* T, int ldt, double* B, int ldb, hipStream_t st) {
+  return dtrsm_left_upper(M, N, T, ldt, B, ldb, st);
+}
+static int trsm_t(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st) {
+  return strsm_left_upper(M, N, T, ldt, B, ldb, st);
 }
 
 void backtrans_plan(ArenaPlan& p, int m, int n) {
@@ -110,31 +128,31 @@ void backtrans_plan(ArenaPlan& p, int m, int n) {
 // Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C:
 // Z = W^T C, Z = T Z (triangular solve with T^{-1}), C -= W Z.
 // left: w_k = A[k:m, k];  right: w_k = A[k, k+1:n] acting on rows k+1..n-1.
-static int apply_blocks(bool left, int m, int n, const double* A, int lda, int nref, double* C,
-                        int ldc, int ncol, double* G, double* Ti, double* Z, double* Wd,
-                        hipStream_t st) {
+template <typename T>
+static int apply_blocks(bool left, int m, int n, const T* A, int lda, int nref, T* C, int ldc,
+                        int ncol, T* G, T* Ti, T* Z, T* Wd, hipStream_t st) {
   const int nblk = (int)ceil_div(nref, NBB);
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int k0 = bi * NBB;
     const int nb = std::min(NBB, nref - k0);
     const int rows = left ? m - k0 : n - k0 - 1;
     if (rows <= 0) continue;
-    double* Csub = left ? C + k0 : C + k0 + 1;
+    T* Csub = left ? C + k0 : C + k0 + 1;
     dim3 gp((unsigned)ceil_div(rows, 32), (unsigned)ceil_div(nb, 32));
     if (left)
-      hipLaunchKernelGGL(bt_pack<false>, gp, dim3(32, 8), 0, st, rows, nb,
+      hipLaunchKernelGGL((bt_pack<false, T>), gp, dim3(32, 8), 0, st, rows, nb,
                          A + k0 + (int64_t)k0 * lda, lda, Wd, rows);
     else
-      hipLaunchKernelGGL(bt_pack<true>, gp, dim3(32, 8), 0, st, rows, nb,
+      hipLaunchKernelGGL((bt_pack<true, T>), gp, dim3(32, 8), 0, st, rows, nb,
                          A + k0 + (int64_t)(k0 + 1) * lda, lda, Wd, rows);
     DDC_LAUNCHED();
-    DDC_TRY(dgemm('T', 'N', nb, nb, rows, 1.0, Wd, rows, Wd, rows, 0.0, G, nb, st));
-    hipLaunchKernelGGL(bt_tinv, dim3((unsigned)ceil_div(nb * nb, 256)), dim3(256), 0, st, nb, G,
-                       Ti);
+    DDC_TRY(gemm_t('T', 'N', nb, nb, rows, T(1), Wd, rows, Wd, rows, T(0), G, nb, st));
+    hipLaunchKernelGGL(bt_tinv<T>, dim3((unsigned)ceil_div(nb * nb, 256)), dim3(256), 0, st, nb,
+                       G, Ti);
     DDC_LAUNCHED();
-    DDC_TRY(dgemm('T', 'N', nb, ncol, rows, 1.0, Wd, rows, Csub, ldc, 0.0, Z, nb, st));
-    DDC_TRY(dtrsm_left_upper(nb, ncol, Ti, nb, Z, nb, st));
-    DDC_TRY(dgemm('N', 'N', rows, ncol, nb, -1.0, Wd, rows, Z, nb, 1.0, Csub, ldc, st));
+    DDC_TRY(gemm_t('T', 'N', nb, ncol, rows, T(1), Wd, rows, Csub, ldc, T(0), Z, nb, st));
+    DDC_TRY(trsm_t(nb, ncol, Ti, nb, Z, nb, st));
+    DDC_TRY(gemm_t('N', 'N', rows, ncol, nb, T(-1), Wd, rows, Z, nb, T(1), Csub, ldc, st));
   }
   return DDC_OK;
 }
@@ -150,15 +168,40 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   if (!Wd) return DDC_E_NOMEM;
   dim3 tb(32, 8);
   dim3 gu((unsigned)ceil_div(n, 32), (unsigned)ceil_div(m, 32));
-  hipLaunchKernelGGL(bt_unpack, gu, tb, 0, st, n, m, Ut, ldb, U, ldu);
+  hipLaunchKernelGGL(bt_unpack<double>, gu, tb, 0, st, n, m, Ut, ldb, U, ldu);
   DDC_LAUNCHED();
   dim3 gv((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
-  hipLaunchKernelGGL(bt_unpack, gv, tb, 0, st, n, n, Vt, ldb, V, ldv);
+  hipLaunchKernelGGL(bt_unpack<double>, gv, tb, 0, st, n, n, Vt, ldb, V, ldv);
   DDC_LAUNCHED();
   // left: reflectors 0..n-1 (the last one is the 1 x 1 sign flip when m == n)
   DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st));
   // right: reflectors 0..n-3 (row n-2 holds no reflection)
   if (n > 2) DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, Ti, Z, Wd, st));
+  return DDC_OK;
+}
+
+void backtrans_plan_f32(ArenaPlan& p, int n) {
+  p.add<float>((size_t)NBB * NBB * 2);
+  p.add<float>((size_t)NBB * n);
+  p.add<float>((size_t)n * NBB);
+}
+
+// FP32 arithmetic throughout (the mixed-precision form refines the result).
+int backtrans_run_f32(int n, const float* A32, int lda, const double* Ut, const double* Vt,
+                      int ldb, float* U32, float* V32, Arena& ws, hipStream_t st) {
+  if (n < 1) return DDC_E_ARG;
+  float* G = ws.take<float>((size_t)NBB * NBB * 2);
+  float* Ti = G ? G + NBB * NBB : nullptr;
+  float* Z = ws.take<float>((size_t)NBB * n);
+  float* Wd = ws.take<float>((size_t)n * NBB);
+  if (!Wd) return DDC_E_NOMEM;
+  dim3 tb(32, 8);
+  dim3 g((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
+  hipLaunchKernelGGL(bt_unpack<float>, g, tb, 0, st, n, n, Ut, ldb, U32, n);
+  hipLaunchKernelGGL(bt_unpack<float>, g, tb, 0, st, n, n, Vt, ldb, V32, n);
+  DDC_LAUNCHED();
+  DDC_TRY(apply_blocks<float>(true, n, n, A32, lda, n, U32, n, n, G, Ti, Z, Wd, st));
+  if (n > 2) DDC_TRY(apply_blocks<float>(false, n, n, A32, lda, n - 2, V32, n, n, G, Ti, Z, Wd, st));
   return DDC_OK;
 }
 

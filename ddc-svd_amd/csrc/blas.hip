@@ -95,4 +95,16 @@ int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
   return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_dtrsm", s);
 }
 
+int strsm_left_upper(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st) {
+  if (M <= 0 || N <= 0) return DDC_OK;
+  std::lock_guard<std::mutex> lk(g_blas_mu);
+  rocblas_handle h;
+  DDC_TRY(blas_handle(st, &h));
+  const float one = 1.0f;
+  rocblas_status s = rocblas_strsm(h, rocblas_side_left, rocblas_fill_upper,
+                                   rocblas_operation_none, rocblas_diagonal_non_unit, M, N, &one,
+                                   T, ldt, B, ldb);
+  return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_strsm", s);
+}
+
 }  // namespace ddc

@@ -129,6 +129,11 @@ void backtrans_plan(ArenaPlan& p, int m, int n);
 int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
                   const double* Vt, int ldb, double* U, int ldu, double* V,
                   int ldv, Arena& ws, hipStream_t st);
+// The same for m = n in FP32 arithmetic: reflectors stored as float in A32,
+// the bidiagonal vectors Ut / Vt (FP64) rounded to float; U32, V32 (ld n).
+void backtrans_plan_f32(ArenaPlan& p, int n);
+int backtrans_run_f32(int n, const float* A32, int lda, const double* Ut, const double* Vt,
+                      int ldb, float* U32, float* V32, Arena& ws, hipStream_t st);
 
 // Explicit Q = H_0 ... H_{n-1} (left, m x m) or P = G_0 ... G_{n-3} (right,
 // n x n) from the reflectors stored in A (m >= n).
@@ -161,6 +166,7 @@ int gemm_bf16_f32(char ta, char tb, int M, int N, int K, float alpha, const void
 // B <- T^{-1} B for the M x M upper triangular T (non-unit diagonal).
 int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
                      hipStream_t st);
+int strsm_left_upper(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st);
 
 // Deterministic wave64 sum (fixed butterfly order).
 __device__ __forceinline__ double wave_sum(double v) {

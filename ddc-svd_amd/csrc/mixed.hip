@@ -7,8 +7,9 @@
 //      plain path), vector arithmetic in FP64, and the blocked trailing
 //      update in FP32 or BF16 on the matrix cores with FP32 accumulation;
 //   2. singular values and bidiagonal vectors in FP64 (divide and conquer,
-//      twisted factorizations), back-transformation in FP64 with the FP32
-//      reflectors widened to FP64;
+//      twisted factorizations), back-transformation in FP32 on the matrix
+//      cores (its rounding is of the order of the FP32 reflectors' own, and
+//      the refinement removes both), U and V widened to FP64;
 //   3. refinement steps in FP64 (Ogita and Aishima, "Iterative refinement
 //      for singular value decomposition based on matrix multiplication",
 //      J. Comput. Appl. Math. 369 (2020)): with T = U^T A V, R = I - U^T U,
@@ -37,10 +38,14 @@ __global__ void mx_to_f32(int rows, int cols, const double* A, int lda, float* B
   }
 }
 
-__global__ void mx_to_f64(int64_t count, const float* A, double* B) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
-       e += (int64_t)gridDim.x * blockDim.x)
-    B[e] = (double)A[e];
+// B (n x n, ld ldb, double) = A (n x n, ld n, float).
+__global__ void mx_to_f64(int n, const float* A, double* B, int ldb) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / n, i = e - j * n;
+    B[i + j * ldb] = (double)A[e];
+  }
 }
 
 // Partial sums of (delta_ij - M_ij)^2 over the n x n matrix M (ld n), one
@@ -134,7 +139,7 @@ void mixed_plan(ArenaPlan& p, int n, int upd) {
   bidiag_plan_f32(p, n, n, lda_p, upd);
   dc_plan(p, n);
   twisted_plan(p, n);
-  backtrans_plan(p, n, n);
+  backtrans_plan_f32(p, n);
 }
 
 static double host_defect(int n, const double* M, double* part, hipStream_t st, int& rc) {
@@ -194,12 +199,18 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
   DDC_TRY(dc_run(n, b.alpha, b.beta, b.sigma, ws, st));
   DDC_TRY(twisted_run(n, b.alpha, b.beta, b.sigma, b.Vt, b.Ut, b.ldb, ws, st));
   DDC_HIP(hipEventRecord(ev[3], st));
-  hipLaunchKernelGGL(mx_to_f64, dim3(gconv), dim3(256), 0, st, tot, b.A32, b.Ap);
-  DDC_LAUNCHED();
+  // back-transformation in FP32 (the refinement works from FP32-accurate U,
+  // V either way: the reflectors are FP32), into the storage of Ap
+  float* U32 = reinterpret_cast<float*>(b.Ap);
+  float* V32 = U32 + (size_t)n * n;
+  DDC_TRY(backtrans_run_f32(n, b.A32, b.lda_p, b.Ut, b.Vt, b.ldb, U32, V32, ws, st));
   double* Uc = steps > 0 ? b.U2 : U;
   double* Vc = steps > 0 ? b.V2 : V;
   const int lduc = steps > 0 ? n : ldu, ldvc = steps > 0 ? n : ldv;
-  DDC_TRY(backtrans_run(n, n, b.Ap, b.lda_p, b.Ut, b.Vt, b.ldb, Uc, lduc, Vc, ldvc, ws, st));
+  const unsigned gsq = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 8192);
+  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, U32, Uc, lduc);
+  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, V32, Vc, ldvc);
+  DDC_LAUNCHED();
   DDC_HIP(hipEventRecord(ev[4], st));
 
   double defect_u = 0.0, defect_v = 0.0, dsig = 0.0;
