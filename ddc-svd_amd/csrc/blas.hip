@@ -95,6 +95,17 @@ int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
   return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_dtrsm", s);
 }
 
+int dsyrk_upper_t(int N, int K, const double* A, int lda, double* C, int ldc, hipStream_t st) {
+  if (N <= 0) return DDC_OK;
+  std::lock_guard<std::mutex> lk(g_blas_mu);
+  rocblas_handle h;
+  DDC_TRY(blas_handle(st, &h));
+  const double one = 1.0, zero = 0.0;
+  rocblas_status s = rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, N, K, &one,
+                                   A, lda, &zero, C, ldc);
+  return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_dsyrk", s);
+}
+
 int strsm_left_upper(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st) {
   if (M <= 0 || N <= 0) return DDC_OK;
   std::lock_guard<std::mutex> lk(g_blas_mu);

@@ -167,6 +167,8 @@ int gemm_bf16_f32(char ta, char tb, int M, int N, int K, float alpha, const void
 int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
                      hipStream_t st);
 int strsm_left_upper(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st);
+// Upper triangle of C (N x N) = A^T A for the K x N matrix A.
+int dsyrk_upper_t(int N, int K, const double* A, int lda, double* C, int ldc, hipStream_t st);
 
 // Deterministic wave64 sum (fixed butterfly order).
 __device__ __forceinline__ double wave_sum(double v) {

@@ -18,7 +18,8 @@
 //      and for i != j the 2 x 2 system of the first-order conditions
 //         F + F^T = R,  G + G^T = S,  T + F^T Sigma + Sigma G = Sigma
 //      gives F_ij, G_ij; then U <- U (I + F), V <- V (I + G).
-//      Each step is six FP64 GEMMs (library, MFMA) and two small kernels.
+//      Each step is four FP64 GEMMs and two SYRKs (library, MFMA) and a
+//      few small kernels.
 #include <algorithm>
 #include <vector>
 
@@ -65,6 +66,26 @@ __global__ __launch_bounds__(256) void mx_defect(int n, const double* M, double*
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Lower triangle of the n x n matrix C (ld n) from its upper triangle, one
+// 32 x 32 tile pair per work-group through LDS (both sides coalesced).
+__global__ void mx_symm(int n, double* C) {
+  __shared__ double tile[32][33];
+  const int bi = blockIdx.x, bj = blockIdx.y;  // lower tile (bi, bj), bi >= bj
+  if (bi < bj) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+  // read the upper tile (bj, bi): rows bj*32.., columns bi*32..
+  for (int q = ty; q < 32; q += 8) {
+    const int r = bj * 32 + tx, c = bi * 32 + q;
+    tile[q][tx] = (r < n && c < n) ? C[r + (int64_t)c * n] : 0.0;
+  }
+  __syncthreads();
+  // write the lower tile (bi, bj): C(i, j) = C(j, i) for i > j
+  for (int q = ty; q < 32; q += 8) {
+    const int i = bi * 32 + tx, j = bj * 32 + q;
+    if (i < n && j < n && i > j) C[i + (int64_t)j * n] = tile[tx][q];
+  }
+}
+
 // sigma_i = T_ii / (1 - (r_ii + s_ii)/2) with r = I - U^T U, s = I - V^T V
 // (P = U^T U, Q = V^T V given).
 __global__ void oa_sigma(int n, const double* T, const double* P, const double* Q, double* sig) {
@@ -74,7 +95,7 @@ __global__ void oa_sigma(int n, const double* T, const double* P, const double* 
   sig[i] = T[d] / (0.5 * (P[d] + Q[d]));
 }
 
-// F and G of the refinement (n x n, ld n).  Off the diagonal:
+// I + F and I + G of the refinement (n x n, ld n).  Off the diagonal:
 //   a = -t_ij - r_ij s_j,  b = -t_ji - s_j q_ij,  d = s_j^2 - s_i^2
 //   f_ij = (-s_j a - s_i b) / d,  g_ij = (-s_i a - s_j b) / d
 // with r = I - U^T U, q = I - V^T V; on it f_ii = r_ii / 2, g_ii = q_ii / 2.
@@ -91,8 +112,8 @@ __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
     const double rij = (i == j ? 1.0 : 0.0) - P[e];
     const double qij = (i == j ? 1.0 : 0.0) - Q[e];
     if (i == j) {
-      F[e] = 0.5 * rij;
-      G[e] = 0.5 * qij;
+      F[e] = 1.0 + 0.5 * rij;
+      G[e] = 1.0 + 0.5 * qij;
       continue;
     }
     const double si = sig[i], sj = sig[j];
@@ -204,23 +225,28 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
   float* U32 = reinterpret_cast<float*>(b.Ap);
   float* V32 = U32 + (size_t)n * n;
   DDC_TRY(backtrans_run_f32(n, b.A32, b.lda_p, b.Ut, b.Vt, b.ldb, U32, V32, ws, st));
-  double* Uc = steps > 0 ? b.U2 : U;
-  double* Vc = steps > 0 ? b.V2 : V;
-  const int lduc = steps > 0 ? n : ldu, ldvc = steps > 0 ? n : ldv;
+  double* U0 = steps > 0 ? b.U2 : U;
+  double* V0 = steps > 0 ? b.V2 : V;
+  const int ldu0 = steps > 0 ? n : ldu, ldv0 = steps > 0 ? n : ldv;
   const unsigned gsq = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 8192);
-  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, U32, Uc, lduc);
-  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, V32, Vc, ldvc);
+  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, U32, U0, ldu0);
+  hipLaunchKernelGGL(mx_to_f64, dim3(gsq), dim3(256), 0, st, n, V32, V0, ldv0);
   DDC_LAUNCHED();
   DDC_HIP(hipEventRecord(ev[4], st));
 
   double defect_u = 0.0, defect_v = 0.0, dsig = 0.0;
   std::vector<double> s_old(n), s_new(n);
+  double *Uc = b.U2, *Vc = b.V2, *X1 = b.W, *X2 = b.T;
+  const dim3 gsy((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
   for (int it = 0; it < steps; ++it) {
-    // U, V (ld n) in U2 / V2 on entry
-    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, A, lda, b.V2, n, 0.0, b.W, n, st));
-    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.U2, n, b.W, n, 0.0, b.T, n, st));
-    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.U2, n, b.U2, n, 0.0, b.P, n, st));
-    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, b.V2, n, b.V2, n, 0.0, b.Q, n, st));
+    // current U, V in Uc, Vc (ld n); X1, X2 free
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, A, lda, Vc, n, 0.0, X1, n, st));
+    DDC_TRY(dgemm('T', 'N', n, n, n, 1.0, Uc, n, X1, n, 0.0, X2, n, st));
+    DDC_TRY(dsyrk_upper_t(n, n, Uc, n, b.P, n, st));
+    DDC_TRY(dsyrk_upper_t(n, n, Vc, n, b.Q, n, st));
+    hipLaunchKernelGGL(mx_symm, gsy, dim3(32, 8), 0, st, n, b.P);
+    hipLaunchKernelGGL(mx_symm, gsy, dim3(32, 8), 0, st, n, b.Q);
+    DDC_LAUNCHED();
     if (it == 0 && diag) {
       int rc;
       defect_u = host_defect(n, b.P, b.part, st, rc);
@@ -229,7 +255,7 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
       DDC_TRY(rc);
     }
     DDC_HIP(hipMemcpyAsync(s_old.data(), b.sigma, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    hipLaunchKernelGGL(oa_sigma, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n, b.T, b.P,
+    hipLaunchKernelGGL(oa_sigma, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, n, X2, b.P,
                        b.Q, b.sigma);
     DDC_LAUNCHED();
     DDC_HIP(hipMemcpyAsync(s_new.data(), b.sigma, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -241,25 +267,19 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
     dsig = smax > 0 ? dsig / smax : 0.0;
     const double g = 1e-13 * smax * smax;
     const unsigned gfg = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 16384);
-    hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, b.T, b.P, b.Q, b.sigma, g, b.F,
+    hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, X2, b.P, b.Q, b.sigma, g, b.F,
                        b.G);
     DDC_LAUNCHED();
-    // U <- U + U F, V <- V + V G (new values into W / T, then back)
+    // U <- U (I + F), V <- V (I + G): into the free pair, or the outputs
     const bool last = it + 1 == steps;
-    double* Un = last ? U : b.W;
-    double* Vn = last ? V : b.T;
+    double* Un = last ? U : X1;
+    double* Vn = last ? V : X2;
     const int ldun = last ? ldu : n, ldvn = last ? ldv : n;
-    DDC_HIP(hipMemcpy2DAsync(Un, sizeof(double) * ldun, b.U2, sizeof(double) * n,
-                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
-    DDC_HIP(hipMemcpy2DAsync(Vn, sizeof(double) * ldvn, b.V2, sizeof(double) * n,
-                             sizeof(double) * n, n, hipMemcpyDeviceToDevice, st));
-    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, b.U2, n, b.F, n, 1.0, Un, ldun, st));
-    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, b.V2, n, b.G, n, 1.0, Vn, ldvn, st));
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, Uc, n, b.F, n, 0.0, Un, ldun, st));
+    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, Vc, n, b.G, n, 0.0, Vn, ldvn, st));
     if (!last) {
-      DDC_HIP(hipMemcpyAsync(b.U2, b.W, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
-                             st));
-      DDC_HIP(hipMemcpyAsync(b.V2, b.T, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice,
-                             st));
+      std::swap(Uc, X1);
+      std::swap(Vc, X2);
     }
   }
   DDC_HIP(hipMemcpyAsync(sigma_out, b.sigma, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
