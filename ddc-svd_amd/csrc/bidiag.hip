@@ -171,24 +171,34 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
     }
   }
 
+  // Loads without conditions, so that all CBL * NQ of them are in flight
+  // together: rows past the column (r >= lda) are clamped to the last row
+  // pair and columns past the matrix to its last column; their c entries
+  // are zero or their sums are never stored.
+  int rq[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) rq[q] = min(r0 + 512 * q, a.lda - 2);
+
   if ((int)blockIdx.x < ncb) {
     const int j0 = k + 1 + blockIdx.x * CBL;
     const int jn = min(CBL, a.n - j0);
+    // All CBL * NQ loads are issued before the first product: without the
+    // scheduling barrier the compiler interleaves them one load at a time
+    // (fewer registers, but one load in flight per wave)
+    double2 v[CBL][NQ];
+#pragma unroll
+    for (int c = 0; c < CBL; ++c) {
+      const TS* col = a.A + (int64_t)(j0 + min(c, jn - 1)) * a.lda;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v[c][q] = ld2(col + rq[q]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
     double acc[CBL];
 #pragma unroll
     for (int c = 0; c < CBL; ++c) {
       acc[c] = 0.0;
-      if (c < jn) {
-        const TS* col = a.A + (int64_t)(j0 + c) * a.lda;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int r = r0 + 512 * q;
-          if (r < a.lda) {
-            const double2 v = ld2(col + r);
-            acc[c] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
-          }
-        }
-      }
+      for (int q = 0; q < NQ; ++q) acc[c] += v[c][q].x * cv[2 * q] + v[c][q].y * cv[2 * q + 1];
     }
     // lanes 8c .. 8c+7 end with column c's wave sum
     const double sc = reduce_scatter<double, CBL>(acc);
@@ -228,28 +238,20 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
 #pragma unroll
     for (int e = 0; e < DPW; ++e) {
       acc[e] = 0.0;
-      const int s = q0 + e;
-      if (s < nd) {
-        if (s < a.t) {
-          const TS* col = a.A + (int64_t)(a.p + s) * a.lda;
+      const int s = min(q0 + e, nd - 1);  // clamped: sums past nd are not stored
+      if (s < a.t) {
+        const TS* col = a.A + (int64_t)(a.p + s) * a.lda;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const int r = r0 + 512 * q;
-            if (r < a.lda) {
-              const double2 v = ld2(col + r);
-              acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
-            }
-          }
-        } else {
-          const double* col = a.X + (int64_t)(s - a.t) * a.lda;
+        for (int q = 0; q < NQ; ++q) {
+          const double2 v = ld2(col + rq[q]);
+          acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
+        }
+      } else {
+        const double* col = a.X + (int64_t)(s - a.t) * a.lda;
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) {
-            const int r = r0 + 512 * q;
-            if (r < a.lda) {
-              const double2 v = ld2(col + r);
-              acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
-            }
-          }
+        for (int q = 0; q < NQ; ++q) {
+          const double2 v = ld2(col + rq[q]);
+          acc[e] += v.x * cv[2 * q] + v.y * cv[2 * q + 1];
         }
       }
     }
