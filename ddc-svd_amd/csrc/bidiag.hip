@@ -365,6 +365,17 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   const bool jv = col < jn;
   const int jc = jv ? j0 + col : k + 1;  // a valid column for the loads of idle lanes
 
+  // A row block issues its first eight columns of A0 before the prologue
+  // (rows past the column clamped; unused), so that they stream during it.
+  const bool rowblk = (int)blockIdx.x < nrb;
+  const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
+  const TS* base = a.A + min(r0, a.lda - 2) + (int64_t)j0 * a.lda;
+  double2 v0[8];
+  if (rowblk) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v0[u] = ld2(base + (int64_t)min(u, jn - 1) * a.lda);
+  }
+
   // Every load first.  Thread (col, half) holds Y[j, s], V[j, s] for the
   // panel columns s = half H + i < t.
   double ys[H] = {}, vs[H] = {};
@@ -442,12 +453,18 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   }
   __syncthreads();
 
-  if ((int)blockIdx.x < nrb) {
-    const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
+  if (rowblk) {
     if (r0 < a.lda) {
-      const TS* base = a.A + r0 + (int64_t)j0 * a.lda;
       double h0 = 0.0, h1 = 0.0;
       int q = 0;
+      if (jn >= 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          h0 += v0[u].x * rs[u];
+          h1 += v0[u].y * rs[u];
+        }
+        q = 8;
+      }
       for (; q + 8 <= jn; q += 8) {
         double2 v[8];
 #pragma unroll
