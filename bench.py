@@ -28,7 +28,8 @@ distribution and scaled by the work ratio S(m,n)/S(m/2,n/2) = 8.
 entries U(-1,1)) in FP64 and FP32 against a loop of plain svd_gpu_dev calls,
 the forward + backward throughput of the differentiable SVD at 2048 x 1024,
 and the mixed-precision SVD of an 8192^2 matrix with singular values graded
-from 1 to 1e-6 against the plain FP64 path (errors per decade of sigma).
+from 1 to 1e-6 against the plain FP64 path (errors per decade of sigma), and
+of the plain workload's own N(0,1) matrix (residual, orthogonality, speed).
 """
 from __future__ import annotations
 
@@ -397,6 +398,52 @@ def mixed_form(n: int = 8192, seed: int = 3) -> dict:
         out[name]["stages"] = dg
         progress(f"mixed: {name} done ({1e3 * t:.0f} ms)")
     out["speedup_f32_1step_vs_plain"] = out["plain_fp64"]["ms"] / out["f32_1step"]["ms"]
+    del A, s, U, V
+    out["gaussian"] = mixed_gaussian(n)
+    return out
+
+
+def mixed_gaussian(n: int = 8192, seed: int = 2) -> dict:
+    """The mixed form on the plain workload's own matrix (n x n N(0,1), seed
+    2), whose close singular values come in isolated pairs: residual and
+    orthogonality of each result, and sigma against the plain FP64 path's;
+    with the default refinement bound and with SVD_GPU_MIXED_SEPARATED."""
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+
+    A = ddcsvd.col_major(torch.from_numpy(gen.normal_matrix(n, n, seed)).cuda())
+    torch.cuda.synchronize()
+    eye = torch.eye(n, dtype=torch.float64, device="cuda")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    def quality(s, U, V):
+        res = float(torch.linalg.norm(A - (U * s) @ V.T) / torch.linalg.norm(A))
+        orth = float(max(torch.linalg.norm(U.T @ U - eye), torch.linalg.norm(V.T @ V - eye)))
+        return res, orth
+
+    out = {"config": f"{n}x{n} N(0,1) seed {seed} (the plain workload's matrix); sigma errors "
+                     f"against the plain FP64 result"}
+    (s0, U, V), t = timed(lambda: ddcsvd.svd_torch(A))
+    res, orth = quality(s0, U, V)
+    out["plain_fp64"] = {"ms": 1e3 * t, "residual": res, "orthogonality": orth}
+    smax = float(s0.max())
+    for name, upd, steps, sep in (("f32_1step", "f32", 1, False), ("f32_2steps", "f32", 2, False),
+                                  ("f32_2steps_separated", "f32", 2, True)):
+        (s, U, V, _), t = timed(lambda: ddcsvd.svd_mixed_torch(A, upd, steps, sep))
+        res, orth = quality(s, U, V)
+        out[name] = {"ms": 1e3 * t, "residual": res, "orthogonality": orth,
+                     "max_abs_dsigma_rel_smax": float((s - s0).abs().max()) / smax,
+                     "speedup_vs_plain": out["plain_fp64"]["ms"] / (1e3 * t)}
+        progress(f"mixed gaussian: {name} done ({1e3 * t:.0f} ms, residual {res:.1e})")
     return out
 
 

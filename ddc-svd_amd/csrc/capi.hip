@@ -502,10 +502,13 @@ static int form_host(bool left, int m, int n, const double* A, double* Q) {
 }
 
 static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
-                     double* dV, int ldv, int upd, int steps, double* diag, hipStream_t user) {
+                     double* dV, int ldv, int update, int steps, double* diag, hipStream_t user) {
   DDC_TRY(check_dims(n, n));
+  const int upd = update & 0xff;
+  const int flags = update & ~0xff;
   if (!dA || !dsigma || !dU || !dV || lda < n || ldu < n || ldv < n || steps < 0 || steps > 8 ||
-      (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16)) {
+      (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16) ||
+      (flags & ~SVD_GPU_MIXED_SEPARATED) != 0) {
     set_err("invalid svd_gpu_mixed arguments");
     return DDC_E_ARG;
   }
@@ -528,7 +531,8 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
     set_err("input matrix holds NaN or Inf");
     return DDC_E_NUMERIC;
   }
-  DDC_TRY(mixed_run(n, dA, lda, dsigma, dU, ldu, dV, ldv, upd, steps, diag, ws, st));
+  DDC_TRY(mixed_run(n, dA, lda, dsigma, dU, ldu, dV, ldv, upd,
+                    (flags & SVD_GPU_MIXED_SEPARATED) != 0, steps, diag, ws, st));
   return DDC_OK;
 }
 

@@ -102,11 +102,13 @@ int bidiag_run_f32(int m, int n, float* A, int lda, double* alpha, double* beta,
                    Arena& ws, hipStream_t st);
 
 // Mixed-precision SVD of the n x n FP64 matrix A (device pointers): FP32
-// bidiagonalization with update mode upd, FP64 bidiagonal solve and
-// back-transformation, `steps` FP64 refinement steps.  diag: see mixed.hip.
+// bidiagonalization with update mode upd, FP64 bidiagonal solve, FP32
+// back-transformation, `steps` FP64 refinement steps (separated: the larger
+// bound on the first-order corrections).  diag: see mixed.hip.
 void mixed_plan(ArenaPlan& p, int n, int upd);
 int mixed_run(int n, const double* A, int lda, double* sigma, double* U, int ldu, double* V,
-              int ldv, int upd, int steps, double* diag, Arena& ws, hipStream_t st);
+              int ldv, int upd, bool separated, int steps, double* diag, Arena& ws,
+              hipStream_t st);
 
 // Singular values (ascending) of the n x (n+1) upper bidiagonal with diagonal
 // a[n] and super-diagonal b[n] by divide and conquer.

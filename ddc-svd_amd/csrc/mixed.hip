@@ -27,7 +27,13 @@
 
 namespace ddc {
 
-constexpr double OA_MAXCORR = 1e-3;  // largest first-order correction applied
+// Largest first-order correction applied.  The default suits spectra with
+// long runs of values closer than the FP32 stage's error (their vectors are
+// only re-orthogonalised); the larger bound, chosen by the caller for
+// spectra whose close values come in isolated pairs, corrects those pairs
+// too, and a second step then brings them to FP64 accuracy.
+constexpr double OA_MAXCORR = 1e-3;
+constexpr double OA_MAXCORR_SEPARATED = 1e-2;
 
 // B (rows x cols, ld ldb, float) = A (double); rows rows..ldb-1 are zeroed.
 __global__ void mx_to_f32(int rows, int cols, const double* A, int lda, float* B, int ldb) {
@@ -100,11 +106,11 @@ __global__ void oa_sigma(int n, const double* T, const double* P, const double* 
 //   f_ij = (-s_j a - s_i b) / d,  g_ij = (-s_i a - s_j b) / d
 // with r = I - U^T U, q = I - V^T V; on it f_ii = r_ii / 2, g_ii = q_ii / 2.
 // Pairs closer than the guard, and pairs whose correction is not small
-// (|f_ij| or |g_ij| > OA_MAXCORR: the vectors were not resolved by the
+// (|f_ij| or |g_ij| > maxcorr: the vectors were not resolved by the
 // reduced-precision stage and a first-order step would only mix them), get
 // the orthogonalising half alone.
 __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
-                      const double* sig, double guard2, double* F, double* G) {
+                      const double* sig, double guard2, double maxcorr, double* F, double* G) {
   const int64_t total = (int64_t)n * n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -132,8 +138,8 @@ __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
     const double a2 = -tji - rij * si;
     const double b2 = -tij - si * qij;
     const double f2 = (-si * a2 - sj * b2) / (-d), g2 = (-sj * a2 - si * b2) / (-d);
-    const bool ok = fabs(f) <= OA_MAXCORR && fabs(g) <= OA_MAXCORR &&
-                    fabs(f2) <= OA_MAXCORR && fabs(g2) <= OA_MAXCORR;
+    const bool ok = fabs(f) <= maxcorr && fabs(g) <= maxcorr && fabs(f2) <= maxcorr &&
+                    fabs(g2) <= maxcorr;
     F[e] = ok ? f : 0.5 * rij;
     G[e] = ok ? g : 0.5 * qij;
   }
@@ -185,7 +191,8 @@ static double host_defect(int n, const double* M, double* part, hipStream_t st, 
 // FP32 conversion, in milliseconds; [7] largest |change of sigma| / sigma_max
 // in the last refinement step.
 int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int ldu, double* V,
-              int ldv, int upd, int steps, double* diag, Arena& ws, hipStream_t st) {
+              int ldv, int upd, bool separated, int steps, double* diag, Arena& ws,
+              hipStream_t st) {
   MixedBufs b;
   b.lda_p = (int)round_up(n, 16);
   b.ldb = (int)round_up(n, 16);
@@ -267,8 +274,8 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
     dsig = smax > 0 ? dsig / smax : 0.0;
     const double g = 1e-13 * smax * smax;
     const unsigned gfg = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 16384);
-    hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, X2, b.P, b.Q, b.sigma, g, b.F,
-                       b.G);
+    hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, X2, b.P, b.Q, b.sigma, g,
+                       separated ? OA_MAXCORR_SEPARATED : OA_MAXCORR, b.F, b.G);
     DDC_LAUNCHED();
     // U <- U (I + F), V <- V (I + G): into the free pair, or the outputs
     const bool last = it + 1 == steps;

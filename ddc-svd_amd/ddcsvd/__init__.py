@@ -216,8 +216,13 @@ MIXED_DIAG_KEYS = ["orth_u_before", "orth_v_before", "bidiag_ms", "values_vector
                    "backtransform_ms", "refine_ms", "convert_ms", "last_sigma_change"]
 
 
-def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1):
-    """svd_gpu_mixed on a square CUDA float64 tensor: (sigma, U, V, diag)."""
+MIXED_SEPARATED = 0x100
+
+
+def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1, separated: bool = False):
+    """svd_gpu_mixed on a square CUDA float64 tensor: (sigma, U, V, diag).
+    separated: SVD_GPU_MIXED_SEPARATED (larger first-order corrections, for
+    spectra without long runs of close values)."""
     import torch
     A = col_major(A)
     n = A.shape[0]
@@ -231,7 +236,8 @@ def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1):
     _check(load().svd_gpu_mixed(n, ctypes.c_void_p(A.data_ptr()), A.stride(1),
                                 ctypes.c_void_p(sigma.data_ptr()), ctypes.c_void_p(U.data_ptr()),
                                 U.stride(1), ctypes.c_void_p(V.data_ptr()), V.stride(1),
-                                MIXED_UPDATES[update], refine_steps, _p(diag),
+                                MIXED_UPDATES[update] | (MIXED_SEPARATED if separated else 0),
+                                refine_steps, _p(diag),
                                 ctypes.c_void_p(stream)))
     return sigma, U, V, dict(zip(MIXED_DIAG_KEYS, diag.tolist()))
 

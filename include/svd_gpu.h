@@ -118,13 +118,20 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
  * counterpart: the reference computes in FP64 only, svd_gpu.c:53).  The
  * bidiagonalization stores the matrix and its reflectors in FP32 (vector
  * arithmetic FP64) and runs its blocked trailing update on the matrix cores
- * with FP32 accumulation in the chosen precision; the bidiagonal solve and
- * the back-transformation are FP64; then refine_steps (0..8) FP64
- * refinement steps (Ogita-Aishima, six GEMMs each) restore accuracy where the
- * FP32 stage resolved the singular vectors.  Device pointers as in
- * svd_gpu_dev; results ascending, A not modified.
+ * with FP32 accumulation in the chosen precision; the bidiagonal solve is
+ * FP64 and the back-transformation FP32 on the matrix cores; then
+ * refine_steps (0..8) FP64 refinement steps (Ogita-Aishima: four GEMMs and
+ * two SYRKs each) restore accuracy where the FP32 stage resolved the
+ * singular vectors.  Device pointers as in svd_gpu_dev; results ascending,
+ * A not modified.
  *   update       SVD_GPU_MIXED_F32, _BF16X3 (BF16 products on the two-term
- *                BF16 split: hi*hi + hi*lo + lo*hi) or _BF16 (one product)
+ *                BF16 split: hi*hi + hi*lo + lo*hi) or _BF16 (one product),
+ *                optionally ORed with SVD_GPU_MIXED_SEPARATED: first-order
+ *                corrections up to 1e-2 instead of 1e-3.  For spectra whose
+ *                close values come in isolated pairs (e.g. Gaussian
+ *                matrices) two steps then reach FP64 accuracy; spectra with
+ *                long runs of values closer than the FP32 stage's error
+ *                (e.g. geometrically graded ones) lose orthogonality with it
  *   diag         optional host array of 8: [0] ||I-U^T U||_F, [1] ||I-V^T V||_F
  *                before refinement, [2] bidiagonalization, [3] values and
  *                vectors, [4] back-transformation, [5] refinement, [6] FP32
@@ -135,6 +142,7 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
 #define SVD_GPU_MIXED_F32 1
 #define SVD_GPU_MIXED_BF16X3 2
 #define SVD_GPU_MIXED_BF16 3
+#define SVD_GPU_MIXED_SEPARATED 0x100
 int svd_gpu_mixed(int n, const double *dA, int lda, double *dsigma, double *dU,
                   int ldu, double *dV, int ldv, int update, int refine_steps,
                   double *diag, void *stream);

@@ -1,6 +1,6 @@
 """GPU tests of the mixed-precision SVD (svd_gpu_mixed): FP32-storage
 bidiagonalization with an FP32 / split-BF16 / BF16 trailing update, FP64
-bidiagonal solve, FP64 refinement steps.
+bidiagonal solve, FP32 back-transformation, FP64 refinement steps.
 
 Bounds (measured behaviour with margin; the FP64 reference is LAPACK on the
 host for the Gaussian case and the exact singular values by construction for
@@ -17,6 +17,9 @@ the graded case):
                                  is below its backward error)
   split-BF16 update, 1 step:     |dsigma|/sigma_max <= 1e-6
   BF16 update, no refinement:    |dsigma|/sigma_max <= 1e-2
+  Gaussian 1024^2, FP32 update, SVD_GPU_MIXED_SEPARATED, 2 steps:
+                                 |dsigma|/sigma_max <= 1e-13, residual <= 1e-12,
+                                 orthogonality <= 1e-11
 """
 import ctypes
 
@@ -87,6 +90,25 @@ def test_mixed_bf16_updates(ddcsvd, update, steps, bound):
     assert np.all(np.isfinite(s)) and e <= bound
 
 
+def test_mixed_separated_flag_reaches_fp64_accuracy(ddcsvd):
+    # Gaussian spectrum: its close values come in isolated pairs, and the
+    # larger correction bound resolves them in two steps
+    n = 1024
+    gpu_case(f"mixed separated {n}")
+    A = gen.normal_matrix(n, n, seed=7)
+    sl = np.sort(np.linalg.svd(A, compute_uv=False))
+    At = torch.from_numpy(np.asfortranarray(A)).cuda()
+    s, U, V, _ = ddcsvd.svd_mixed_torch(At, "f32", 2, separated=True)
+    torch.cuda.synchronize()
+    s, U, V = s.cpu().numpy(), U.cpu().numpy(), V.cpu().numpy()
+    e = np.max(np.abs(s - sl)) / sl[-1]
+    res = np.linalg.norm(A - (U * s) @ V.T) / np.linalg.norm(A)
+    orth = max(np.linalg.norm(U.T @ U - np.eye(n)), np.linalg.norm(V.T @ V - np.eye(n)))
+    print(f"[mixed] separated 2 steps: sigma {e:.2e} residual {res:.2e} orth {orth:.2e}",
+          flush=True)
+    assert e <= 1e-13 and res <= 1e-12 and orth <= 1e-11
+
+
 def test_mixed_rejects_bad_arguments(ddcsvd):
     gpu_case("mixed argument checks")
     lib = ddcsvd.load()
@@ -97,3 +119,5 @@ def test_mixed_rejects_bad_arguments(ddcsvd):
     assert lib.svd_gpu_mixed(0, p(A), 8, p(s), p(A), 8, p(A), 8, 1, 1, None, None) == ddcsvd.E_ARG
     assert lib.svd_gpu_mixed(8, p(A), 4, p(s), p(A), 8, p(A), 8, 1, 1, None, None) == ddcsvd.E_ARG
     assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1, 9, None, None) == ddcsvd.E_ARG
+    assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1 | 0x200, 1, None,
+                             None) == ddcsvd.E_ARG
