@@ -109,6 +109,18 @@ def test_mixed_separated_flag_reaches_fp64_accuracy(ddcsvd):
     assert e <= 1e-13 and res <= 1e-12 and orth <= 1e-11
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 37, 777])
+def test_mixed_odd_sizes(ddcsvd, n):
+    # shapes off every tile boundary of the FP32 stages, two refinement steps
+    gpu_case(f"mixed odd size {n}")
+    A = gen.normal_matrix(n, n, seed=11 + n)
+    sl = np.sort(np.linalg.svd(A, compute_uv=False))
+    s, res, orth, _ = _run(ddcsvd, A, "f32", 2)
+    e = np.max(np.abs(s - sl)) / sl[-1]
+    print(f"[mixed] n={n}: sigma {e:.2e} residual {res:.2e} orth {orth:.2e}", flush=True)
+    assert np.all(np.isfinite(s)) and e <= 1e-12 and res <= 1e-9 and orth <= 1e-9
+
+
 def test_mixed_rejects_bad_arguments(ddcsvd):
     gpu_case("mixed argument checks")
     lib = ddcsvd.load()
