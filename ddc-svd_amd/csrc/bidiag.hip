@@ -78,6 +78,23 @@ __device__ __forceinline__ double2 ld2(const float* p) {
   return make_double2((double)v.x, (double)v.y);
 }
 
+// Two consecutive stored entries, raw (no conversion: the loads of a batch
+// can all be in flight before the first use).
+template <typename TS>
+struct Raw2;
+template <>
+struct Raw2<double> {
+  using T = double2;
+};
+template <>
+struct Raw2<float> {
+  using T = float2;
+};
+template <typename TS>
+__device__ __forceinline__ typename Raw2<TS>::T ldraw2(const TS* p) {
+  return *reinterpret_cast<const typename Raw2<TS>::T*>(p);
+}
+
 __device__ __forceinline__ double block_sum256(double v, double* red) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -185,12 +202,12 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
     // All CBL * NQ loads are issued before the first product: without the
     // scheduling barrier the compiler interleaves them one load at a time
     // (fewer registers, but one load in flight per wave)
-    double2 v[CBL][NQ];
+    typename Raw2<TS>::T v[CBL][NQ];
 #pragma unroll
     for (int c = 0; c < CBL; ++c) {
       const TS* col = a.A + (int64_t)(j0 + min(c, jn - 1)) * a.lda;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) v[c][q] = ld2(col + rq[q]);
+      for (int q = 0; q < NQ; ++q) v[c][q] = ldraw2(col + rq[q]);
     }
     __builtin_amdgcn_sched_barrier(0);
     double acc[CBL];
@@ -198,7 +215,8 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
     for (int c = 0; c < CBL; ++c) {
       acc[c] = 0.0;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[c] += v[c][q].x * cv[2 * q] + v[c][q].y * cv[2 * q + 1];
+      for (int q = 0; q < NQ; ++q)
+        acc[c] += (double)v[c][q].x * cv[2 * q] + (double)v[c][q].y * cv[2 * q + 1];
     }
     // lanes 8c .. 8c+7 end with column c's wave sum
     const double sc = reduce_scatter<double, CBL>(acc);
