@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
 
   // The product is taken with the raw column c (rows k..m-1), not with the
   // reflector u = (c + s|c| e_k) / rl: no load waits for the norm of c, and
-  // left_fin applies  A0^T u = (A0^T c + s|c| A0[k, :]) / rl.
+  // the left finalization (in bd_right_gemv, or bd_left_fin at k = n-2)
+  // applies  A0^T u = (A0^T c + s|c| A0[k, :]) / rl.
   // rows of this thread: r0 + 512 q + {0, 1}, q < NQ (coalesced pairs)
   const int r0 = rb0 + blockIdx.y * (512 * NQ) + 2 * tid;
   double cv[2 * NQ];
@@ -283,6 +284,8 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
 }
 
 // left_fin: one thread per trailing column j in [k+1, n), one wave per group.
+// Launched only at k = n-2 (no right GEMV follows); every other step does
+// this work in bd_right_gemv's prologue.
 template <typename TS>
 __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
   __shared__ double dd[2 * NB];  // u_s^T u, x_s^T u
