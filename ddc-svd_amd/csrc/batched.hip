@@ -702,11 +702,11 @@ struct svd_gpu_batch {
   int count = 0, dtype = 0, cus = 0, dev = 0;
   std::vector<int> m, n;
   std::vector<long long> offA, offS, offU, offV;
-  std::vector<int> order, sample;
+  std::vector<int> order;
   double work = 0.0;
   ddc::BatchItem* d_items = nullptr;
-  int* d_order = nullptr;   // full order, then the tuning sample
-  int* d_counter = nullptr;  // [0] run queue, [1] tuning queue
+  int* d_order = nullptr;   // processing order
+  int* d_counter = nullptr;  // [0] queue head
   void* d_work = nullptr;
   size_t work_elems_a = 0, work_elems_v = 0;
   ddc::BjConfig cfg;
@@ -723,12 +723,12 @@ namespace ddc {
 
 template <typename T>
 static BjArgs<T> bj_args(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* dV,
-                         int* info, bool sample) {
+                         int* info) {
   BjArgs<T> a;
-  a.count = sample ? (int)p->sample.size() : p->count;
+  a.count = p->count;
   a.items = p->d_items;
-  a.order = p->d_order + (sample ? p->count : 0);
-  a.counter = p->d_counter + (sample ? 1 : 0);
+  a.order = p->d_order;
+  a.counter = p->d_counter;
   a.A = (const T*)dA;
   a.S = (T*)dS;
   a.U = (T*)dU;
@@ -739,14 +739,14 @@ static BjArgs<T> bj_args(svd_gpu_batch* p, const void* dA, void* dS, void* dU, v
   return a;
 }
 
-static int bj_enqueue(svd_gpu_batch* p, const BjConfig& c, bool sample, const void* dA,
-                      void* dS, void* dU, void* dV, int* info, hipStream_t st) {
-  int* ctr = p->d_counter + (sample ? 1 : 0);
+static int bj_enqueue(svd_gpu_batch* p, const BjConfig& c, const void* dA, void* dS, void* dU,
+                      void* dV, int* info, hipStream_t st) {
+  int* ctr = p->d_counter;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, ctr, 0, 1);
   DDC_LAUNCHED();
   if (p->dtype == SVD_GPU_F64)
-    return bj_launch<double>(c, p->cus, bj_args<double>(p, dA, dS, dU, dV, info, sample), st);
-  return bj_launch<float>(c, p->cus, bj_args<float>(p, dA, dS, dU, dV, info, sample), st);
+    return bj_launch<double>(c, p->cus, bj_args<double>(p, dA, dS, dU, dV, info), st);
+  return bj_launch<float>(c, p->cus, bj_args<float>(p, dA, dS, dU, dV, info), st);
 }
 
 static std::tuple<int, int, int> tune_key(const svd_gpu_batch* p) {
@@ -754,14 +754,13 @@ static std::tuple<int, int, int> tune_key(const svd_gpu_batch* p) {
           (int)std::lround(std::log2(p->work + 1.0))};
 }
 
-// Milliseconds of one launch of configuration c on the tuning sample or the
-// whole batch (best of 2).
-static int bj_time(svd_gpu_batch* p, const BjConfig& c, bool sample, const void* dA, void* dS,
-                   void* dU, void* dV, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float& ms) {
+// Milliseconds of one launch of configuration c on the whole batch (best of 2).
+static int bj_time(svd_gpu_batch* p, const BjConfig& c, const void* dA, void* dS, void* dU,
+                   void* dV, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float& ms) {
   ms = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {
     DDC_HIP(hipEventRecord(e0, st));
-    DDC_TRY(bj_enqueue(p, c, sample, dA, dS, dU, dV, nullptr, st));
+    DDC_TRY(bj_enqueue(p, c, dA, dS, dU, dV, nullptr, st));
     DDC_HIP(hipEventRecord(e1, st));
     DDC_HIP(hipEventSynchronize(e1));
     float t = 0.f;
@@ -772,11 +771,10 @@ static int bj_time(svd_gpu_batch* p, const BjConfig& c, bool sample, const void*
 }
 
 // The autotuner: times (variant, workgroups per CU) with a middle
-// cooperation threshold, then the thresholds for the best pair, on the
-// sample (every second matrix).  The sample keeps the batch's largest
-// matrices, whose latency floors the makespan, but only half of the bulk, so
-// its ranking can differ from the whole batch's: the three best
-// configurations are timed again on the whole batch, which decides.
+// cooperation threshold, then the thresholds for the best pair, each on the
+// whole batch.  (A half-size sample was tried: it keeps the largest
+// matrices, whose latency floors the makespan, but only half of the bulk,
+// and ranked configurations 25% apart in the wrong order.)
 static int bj_tune(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* dV,
                    hipStream_t st) {
   hipEvent_t e0, e1;
@@ -787,7 +785,6 @@ static int bj_tune(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* d
   }
   float best = 1e30f;
   BjConfig bc;
-  std::vector<std::pair<float, BjConfig>> seen;
   int rc = DDC_OK;
   for (int v = 0; v < kNumVariants && !rc; ++v)
     for (int wg : kWgsPerCu) {
@@ -796,8 +793,7 @@ static int bj_tune(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* d
       c.wgs = wg;
       c.coop_nt = kCoopNt[0];
       float ms;
-      if ((rc = bj_time(p, c, true, dA, dS, dU, dV, st, e0, e1, ms))) break;
-      seen.push_back({ms, c});
+      if ((rc = bj_time(p, c, dA, dS, dU, dV, st, e0, e1, ms))) break;
       if (ms < best) {
         best = ms;
         bc = c;
@@ -808,26 +804,10 @@ static int bj_tune(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* d
     BjConfig c = bc;
     c.coop_nt = ct;
     float ms;
-    if ((rc = bj_time(p, c, true, dA, dS, dU, dV, st, e0, e1, ms))) break;
-    seen.push_back({ms, c});
+    if ((rc = bj_time(p, c, dA, dS, dU, dV, st, e0, e1, ms))) break;
     if (ms < best) {
       best = ms;
       bc = c;
-    }
-  }
-  if (!rc && p->sample.size() < (size_t)p->count) {
-    std::stable_sort(seen.begin(), seen.end(),
-                     [](const std::pair<float, BjConfig>& x, const std::pair<float, BjConfig>& y) {
-                       return x.first < y.first;
-                     });
-    float bfull = 1e30f;
-    for (size_t i = 0; i < seen.size() && i < 3 && !rc; ++i) {
-      float ms;
-      if ((rc = bj_time(p, seen[i].second, false, dA, dS, dU, dV, st, e0, e1, ms))) break;
-      if (ms < bfull) {
-        bfull = ms;
-        bc = seen[i].second;
-      }
     }
   }
   (void)hipEventDestroy(e0);
@@ -895,14 +875,11 @@ static int bj_plan(svd_gpu_batch* p, int count, const int* m, const int* n, int 
   std::iota(p->order.begin(), p->order.end(), 0);
   std::stable_sort(p->order.begin(), p->order.end(),
                    [&](int x, int y) { return cost[x] > cost[y]; });
-  // tuning sample: every 2nd matrix of the order, so it has the batch's mix
-  p->sample.clear();
-  for (int i = 0; i < count; i += 2) p->sample.push_back(p->order[i]);
 
   DDC_HIP(hipGetDevice(&p->dev));
   DDC_HIP(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->dev));
   if (p->cus < 1) p->cus = 1;
-  const size_t order_len = p->order.size() + p->sample.size();
+  const size_t order_len = p->order.size();
   if (hipMalloc(&p->d_items, sizeof(BatchItem) * count) != hipSuccess ||
       hipMalloc(&p->d_order, sizeof(int) * order_len) != hipSuccess ||
       hipMalloc(&p->d_counter, sizeof(int) * 64) != hipSuccess ||
@@ -912,7 +889,6 @@ static int bj_plan(svd_gpu_batch* p, int count, const int* m, const int* n, int 
     return DDC_E_NOMEM;
   }
   std::vector<int> ord(p->order);
-  ord.insert(ord.end(), p->sample.begin(), p->sample.end());
   DDC_HIP(hipMemcpy(p->d_items, items.data(), sizeof(BatchItem) * count, hipMemcpyHostToDevice));
   DDC_HIP(hipMemcpy(p->d_order, ord.data(), sizeof(int) * order_len, hipMemcpyHostToDevice));
   DDC_HIP(hipMemset(p->d_counter, 0, sizeof(int) * 64));
@@ -936,7 +912,7 @@ static int bj_run(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* dV
     p->exec = nullptr;
     p->graph = nullptr;
     DDC_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = bj_enqueue(p, p->cfg, false, dA, dS, dU, dV, info, p->cap);
+    const int rc = bj_enqueue(p, p->cfg, dA, dS, dU, dV, info, p->cap);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(p->cap, &gr);
     if (rc) {
