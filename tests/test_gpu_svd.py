@@ -67,8 +67,10 @@ def check_svd(A, s, U, V, sig_tol=1e-12, vec=True):
 
 # ---------------------------------------------------------------- bidiagonalization
 
+# (2500, 200) and (4200, 130): several 2048-row chunks of the left GEMV and
+# several 512-row blocks of the right GEMV
 BIDIAG_SHAPES = [(1, 1), (5, 1), (2, 2), (3, 2), (3, 3), (8, 3), (33, 33), (64, 64),
-                 (96, 64), (100, 70), (130, 97), (300, 300)]
+                 (96, 64), (100, 70), (130, 97), (300, 300), (2500, 200), (4200, 130)]
 
 
 @pytest.mark.parametrize("shape", BIDIAG_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
@@ -225,7 +227,7 @@ def test_bdsvd_vectors_vs_golden_twisted(ddcsvd):
 # ---------------------------------------------------------------- full svd_gpu
 
 SVD_SHAPES = [(1, 1), (5, 1), (1, 5), (2, 2), (3, 3), (7, 5), (5, 7), (33, 31), (64, 64),
-              (96, 64), (64, 96), (130, 100), (300, 300), (512, 256)]
+              (96, 64), (64, 96), (130, 100), (300, 300), (512, 256), (4200, 300), (300, 4200)]
 
 
 @pytest.mark.parametrize("shape", SVD_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
