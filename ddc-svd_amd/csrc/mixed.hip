@@ -18,8 +18,8 @@
 //      and for i != j the 2 x 2 system of the first-order conditions
 //         F + F^T = R,  G + G^T = S,  T + F^T Sigma + Sigma G = Sigma
 //      gives F_ij, G_ij; then U <- U (I + F), V <- V (I + G).
-//      Each step is four FP64 GEMMs and two SYRKs (library, MFMA) and a
-//      few small kernels.
+//      Each step is two FP64 GEMMs and two SYRKs (library, MFMA) for T, R
+//      and S, and two FP32 GEMMs for the update products U F and V G.
 #include <algorithm>
 #include <vector>
 
@@ -52,6 +52,25 @@ __global__ void mx_to_f64(int n, const float* A, double* B, int ldb) {
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t j = e / n, i = e - j * n;
     B[i + j * ldb] = (double)A[e];
+  }
+}
+
+// D (n x n, ld n, float) = S (n x n, ld n, double).
+__global__ void mx_round_f32(int n, const double* S, float* D) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    D[e] = (float)S[e];
+}
+
+// Un (ld ldun) = Uc + UF for the n x n matrices Uc (double, ld n) and UF
+// (float, ld n).
+__global__ void mx_add_f32(int n, const double* Uc, const float* UF, double* Un, int ldun) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e / n, i = e - j * n;
+    Un[i + j * ldun] = Uc[e] + (double)UF[e];
   }
 }
 
@@ -101,7 +120,7 @@ __global__ void oa_sigma(int n, const double* T, const double* P, const double* 
   sig[i] = T[d] / (0.5 * (P[d] + Q[d]));
 }
 
-// I + F and I + G of the refinement (n x n, ld n).  Off the diagonal:
+// F and G of the refinement (n x n, ld n).  Off the diagonal:
 //   a = -t_ij - r_ij s_j,  b = -t_ji - s_j q_ij,  d = s_j^2 - s_i^2
 //   f_ij = (-s_j a - s_i b) / d,  g_ij = (-s_i a - s_j b) / d
 // with r = I - U^T U, q = I - V^T V; on it f_ii = r_ii / 2, g_ii = q_ii / 2.
@@ -118,8 +137,8 @@ __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
     const double rij = (i == j ? 1.0 : 0.0) - P[e];
     const double qij = (i == j ? 1.0 : 0.0) - Q[e];
     if (i == j) {
-      F[e] = 1.0 + 0.5 * rij;
-      G[e] = 1.0 + 0.5 * qij;
+      F[e] = 0.5 * rij;
+      G[e] = 0.5 * qij;
       continue;
     }
     const double si = sig[i], sj = sig[j];
@@ -150,6 +169,7 @@ struct MixedBufs {
   float* A32;
   double *Ap, *alpha, *beta, *sigma, *Vt, *Ut;
   double *T, *P, *Q, *W, *F, *G, *U2, *V2, *part;
+  float *U32, *F32, *UF32;  // the update products U F, V G in FP32 (2 n^2 each)
 };
 
 constexpr int MX_PARTS = 1024;
@@ -162,6 +182,7 @@ void mixed_plan(ArenaPlan& p, int n, int upd) {
   for (int i = 0; i < 3; ++i) p.add<double>(n);
   p.add<double>((size_t)ldb * n * 2);
   for (int i = 0; i < 8; ++i) p.add<double>((size_t)n * n);
+  for (int i = 0; i < 3; ++i) p.add<float>((size_t)n * n * 2);
   p.add<double>(MX_PARTS);
   bidiag_plan_f32(p, n, n, lda_p, upd);
   dc_plan(p, n);
@@ -205,6 +226,9 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
   b.Ut = b.Vt ? b.Vt + (size_t)b.ldb * n : nullptr;
   double** mats[8] = {&b.T, &b.P, &b.Q, &b.W, &b.F, &b.G, &b.U2, &b.V2};
   for (auto pm : mats) *pm = ws.take<double>((size_t)n * n);
+  b.U32 = ws.take<float>((size_t)n * n * 2);
+  b.F32 = ws.take<float>((size_t)n * n * 2);
+  b.UF32 = ws.take<float>((size_t)n * n * 2);
   b.part = ws.take<double>(MX_PARTS);
   if (!b.part) return DDC_E_NOMEM;
 
@@ -277,13 +301,28 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
     hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, X2, b.P, b.Q, b.sigma, g,
                        separated ? OA_MAXCORR_SEPARATED : OA_MAXCORR, b.F, b.G);
     DDC_LAUNCHED();
-    // U <- U (I + F), V <- V (I + G): into the free pair, or the outputs
+    // U <- U + U F, V <- V + V G into the free pair, or the outputs.  The
+    // products are FP32 GEMMs: F and G are first-order corrections (entries
+    // at most the correction bound), so the products' rounding, about
+    // 6e-8 |U| |F|, is second order against the step's own error and
+    // vanishes as F does.
     const bool last = it + 1 == steps;
     double* Un = last ? U : X1;
     double* Vn = last ? V : X2;
     const int ldun = last ? ldu : n, ldvn = last ? ldv : n;
-    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, Uc, n, b.F, n, 0.0, Un, ldun, st));
-    DDC_TRY(dgemm('N', 'N', n, n, n, 1.0, Vc, n, b.G, n, 0.0, Vn, ldvn, st));
+    const size_t nn = (size_t)n * n;
+    const unsigned gsq2 = (unsigned)std::min<int64_t>(ceil_div((int64_t)nn, 256), 8192);
+    hipLaunchKernelGGL(mx_round_f32, dim3(gsq2), dim3(256), 0, st, n, Uc, b.U32);
+    hipLaunchKernelGGL(mx_round_f32, dim3(gsq2), dim3(256), 0, st, n, Vc, b.U32 + nn);
+    hipLaunchKernelGGL(mx_round_f32, dim3(gsq2), dim3(256), 0, st, n, b.F, b.F32);
+    hipLaunchKernelGGL(mx_round_f32, dim3(gsq2), dim3(256), 0, st, n, b.G, b.F32 + nn);
+    DDC_LAUNCHED();
+    DDC_TRY(sgemm('N', 'N', n, n, n, 1.0f, b.U32, n, b.F32, n, 0.0f, b.UF32, n, st));
+    DDC_TRY(sgemm('N', 'N', n, n, n, 1.0f, b.U32 + nn, n, b.F32 + nn, n, 0.0f, b.UF32 + nn, n,
+                  st));
+    hipLaunchKernelGGL(mx_add_f32, dim3(gsq2), dim3(256), 0, st, n, Uc, b.UF32, Un, ldun);
+    hipLaunchKernelGGL(mx_add_f32, dim3(gsq2), dim3(256), 0, st, n, Vc, b.UF32 + nn, Vn, ldvn);
+    DDC_LAUNCHED();
     if (!last) {
       std::swap(Uc, X1);
       std::swap(Vc, X2);
