@@ -120,9 +120,9 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
  * arithmetic FP64) and runs its blocked trailing update on the matrix cores
  * with FP32 accumulation in the chosen precision; the bidiagonal solve is
  * FP64 and the back-transformation FP32 on the matrix cores; then
- * refine_steps (0..8) FP64 refinement steps (Ogita-Aishima: four GEMMs and
- * two SYRKs each) restore accuracy where the FP32 stage resolved the
- * singular vectors.  Device pointers as in svd_gpu_dev; results ascending,
+ * refine_steps (0..8) refinement steps (Ogita-Aishima: U^T A V, U^T U and
+ * V^T V in FP64, the update products U F and V G in FP32) restore accuracy
+ * where the FP32 stage resolved the singular vectors.  Device pointers as in svd_gpu_dev; results ascending,
  * A not modified.
  *   update       SVD_GPU_MIXED_F32, _BF16X3 (BF16 products on the two-term
  *                BF16 split: hi*hi + hi*lo + lo*hi) or _BF16 (one product),
