@@ -266,7 +266,8 @@ def backward_form(reps: int = 3) -> dict:
 
 def pmc_pass(m: int, n: int, seed: int) -> int:
     """Workload for one rocprofv3 --pmc pass: three 1 GiB device copies (the
-    calibration kernel of known traffic), then one svd_gpu_dev call."""
+    calibration kernel of known traffic), then the n-1 left-GEMV launches of
+    one bidiagonalization alone (svd_gpu_profile_left_gemv)."""
     import numpy as np
     import torch
 
@@ -280,12 +281,9 @@ def pmc_pass(m: int, n: int, seed: int) -> int:
         y.copy_(x)
     torch.cuda.synchronize()
     del x, y
-    mn = min(m, n)
     A = ddcsvd.col_major(torch.from_numpy(gen.normal_matrix(m, n, seed)).cuda())
-    sigma = torch.empty(mn, dtype=torch.float64, device="cuda")
-    U = ddcsvd.col_major(torch.empty((m, mn), dtype=torch.float64, device="cuda"))
-    V = ddcsvd.col_major(torch.empty((n, mn), dtype=torch.float64, device="cuda"))
-    ddcsvd.svd_dev(A, sigma, U, V, torch.cuda.current_stream().cuda_stream)
+    print(f"pmc pass: {n - 1} left-GEMV launches, {m}x{n}", flush=True)
+    ddcsvd.profile_left_gemv(A, n - 1, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     print("pmc pass done", flush=True)
     return 0
@@ -312,7 +310,16 @@ def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
                 names[did] = r["Kernel_Name"]
         return per, names
 
-    out = {"workload": f"one svd_gpu_dev call, {m}x{n} N(0,1) seed 2"}
+    out = {"workload": f"the {n - 1} left-GEMV launches of one {m}x{n} N(0,1) bidiagonalization "
+                       "(svd_gpu_profile_left_gemv), seed 2",
+           "commands": [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run --output-format csv "
+                        f"-- python3 bench.py --pmc-pass --m {m} --n {n}",
+                        f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o run --output-format csv "
+                        f"-- python3 bench.py --pmc-pass --m {m} --n {n}",
+                        f"python3 bench.py --summarize-pmc {fetch_dir} {write_dir} --m {m} --n {n}"],
+           "calibration": "bytes per counter unit = 1 GiB / the counter's mean over the three "
+                          "1 GiB device copies of the same pass (each reads 1 GiB and writes "
+                          "1 GiB)"}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         per, names = load(d, counter)
         copy = [v for k, v in per.items() if "opy" in names[k] or "elementwise" in names[k]]

@@ -893,6 +893,58 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta, Ar
   return bidiag_drive<double>(m, n, A, lda, alpha, beta, 0, ws, st);
 }
 
+// Measurement: `steps` left-GEMV launches, at column steps k = 0..steps-1,
+// on the panel-start matrix A (m >= n; A is overwritten with reflectors,
+// alpha with their diagonals).  The column of step 0 is c for every step and
+// no other kernel of the step runs, so a counter pass sees the left GEMV's
+// own traffic for the trailing blocks A[k:m, k+1:n].
+int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps, Arena& ws,
+                          hipStream_t st) {
+  if (m < n || n < 2 || lda < m || (lda % 2) != 0 || steps < 1 || steps > n - 1)
+    return DDC_E_ARG;
+  BidiagBufs d = bidiag_dims(m, n, lda);
+  StepArgs<double> a;
+  a.m = m;
+  a.n = n;
+  a.lda = lda;
+  a.ldy = d.ldy;
+  a.A = A;
+  a.alpha = alpha;
+  a.beta = nullptr;
+  a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
+  a.V = a.Y + (size_t)d.ldy * NB;
+  double* UX = ws.take<double>((size_t)lda * NB * 2);
+  a.X = UX ? UX + (size_t)lda * NB : nullptr;
+  a.Ucopy = UX;
+  a.cvec = ws.take<double>(lda);
+  a.rvec = ws.take<double>(d.ldy);
+  a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
+  a.Rpart = ws.take<double>((size_t)d.nccR_max * lda);
+  a.Ldot = ws.take<double>((size_t)d.nchL_max * 2 * NB);
+  a.Rdot = ws.take<double>((size_t)d.nccR_max * DSTRIDE);
+  a.cnorm = ws.take<double>(d.ncn_max);
+  a.rnorm = ws.take<double>(d.nrn_max);
+  if (!a.rnorm) return DDC_E_NOMEM;
+  DDC_HIP(hipMemsetAsync(a.X, 0, sizeof(double) * (size_t)lda * NB, st));
+  a.ncn = (int)ceil_div(m, 256);
+  a.nrn = 0;
+  hipLaunchKernelGGL(bd_init<double>, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
+  DDC_LAUNCHED();
+  for (int k = 0; k < steps; ++k) {
+    a.k = k;
+    a.p = (k / NB) * NB;
+    a.t = k - a.p;
+    const int rb0 = (k / 64) * 64;
+    const int ncb = (int)ceil_div(n - k - 1, CBL);
+    const int ndl = (int)ceil_div(2 * a.t, DPW);
+    const int nchL = (int)ceil_div(m - rb0, RC);
+    hipLaunchKernelGGL((bd_left_gemv<double, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0, st,
+                       a, ncb, rb0);
+    DDC_LAUNCHED();
+  }
+  return DDC_OK;
+}
+
 int bidiag_run_f32(int m, int n, float* A, int lda, double* alpha, double* beta, int upd,
                    Arena& ws, hipStream_t st) {
   if (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16) return DDC_E_ARG;

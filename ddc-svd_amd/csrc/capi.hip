@@ -307,6 +307,33 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
   return DDC_OK;
 }
 
+// Counter-collection workload: a padded copy of dA, then `steps` left-GEMV
+// launches of its bidiagonalization and nothing else.
+static int left_gemv_dev(int m, int n, const double* dA, int lda, int steps, hipStream_t user) {
+  DDC_TRY(check_dims(m, n));
+  if (!dA || m < n || n < 2 || lda < m || steps < 1 || steps > n - 1) {
+    set_err("invalid arguments (need m >= n >= 2, lda >= m, 1 <= steps <= n-1)");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(ctx_init());
+  hipStream_t st = user ? user : g_ctx.st;
+  const int lda_p = (int)round_up(m, 16);
+  ArenaPlan plan;
+  plan.add<double>((size_t)lda_p * n);
+  plan.add<double>(n);
+  bidiag_plan(plan, m, n, lda_p);
+  Arena ws;
+  DDC_TRY(arena_get(plan.bytes, ws));
+  double* Ap = ws.take<double>((size_t)lda_p * n);
+  double* da = ws.take<double>(n);
+  DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
+  DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, dA, sizeof(double) * lda,
+                           sizeof(double) * m, n, hipMemcpyDeviceToDevice, st));
+  DDC_TRY(bidiag_left_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  return DDC_OK;
+}
+
 static int svd_dev(int m, int n, const double* dA, int lda, double* dsigma, double* dU,
                    int ldu, double* dV, int ldv, hipStream_t user) {
   DDC_TRY(check_dims(m, n));
@@ -627,6 +654,13 @@ int svd_gpu_last_timings(double* ms, int count) {
   int k = count < kNumTimes ? count : kNumTimes;
   for (int i = 0; i < k; ++i) ms[i] = g_times[i];
   return k;
+}
+
+int svd_gpu_profile_left_gemv(int m, int n, const double* dA, int lda, int steps,
+                              void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return left_gemv_dev(m, n, dA, lda, steps, (hipStream_t)stream);
 }
 
 int svd_gpu_profile_kernels(int on) {

@@ -88,6 +88,9 @@ struct ArenaPlan {
 // as the reference stores them (bidiag_par.c:34-450), alpha[n], beta[n] with
 // beta[n-1] = 0.  nb is the panel width of the blocked update.
 void bidiag_plan(ArenaPlan& p, int m, int n, int lda);
+// Measurement: `steps` left-GEMV launches alone (A and alpha overwritten).
+int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps, Arena& ws,
+                          hipStream_t st);
 int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
                Arena& ws, hipStream_t st);
 

@@ -81,6 +81,8 @@ def load():
         lib.svd_gpu_last_timings.restype = i
         lib.svd_gpu_profile_kernels.argtypes = [i]
         lib.svd_gpu_profile_kernels.restype = i
+        lib.svd_gpu_profile_left_gemv.argtypes = [i, i, v, i, i, v]
+        lib.svd_gpu_profile_left_gemv.restype = i
         lib.svd_gpu_last_error.argtypes = []
         lib.svd_gpu_last_error.restype = ctypes.c_char_p
         _lib = lib
@@ -115,6 +117,15 @@ def last_timings() -> dict:
 def profile_kernels(on: bool) -> None:
     """Bracket every bidiagonalization GEMV launch with HIP events."""
     load().svd_gpu_profile_kernels(1 if on else 0)
+
+
+def profile_left_gemv(A, steps: int, stream=None) -> None:
+    """svd_gpu_profile_left_gemv on a column-major float64 CUDA matrix: the
+    first `steps` left-GEMV launches of its bidiagonalization, alone."""
+    m, n = A.shape
+    lda = _colmajor_ld(A, "A")
+    _check(load().svd_gpu_profile_left_gemv(m, n, ctypes.c_void_p(A.data_ptr()), lda, steps,
+                                            ctypes.c_void_p(stream)))
 
 
 def svd(A: np.ndarray, raw: bool = False):

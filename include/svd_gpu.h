@@ -216,6 +216,14 @@ int svd_gpu_last_timings(double *ms, int count);
  * svd_gpu_last_timings; costs two event records per launch. */
 int svd_gpu_profile_kernels(int on);
 
+/* Measurement hook (no reference counterpart): the first `steps` left-GEMV
+ * launches (1 <= steps <= n-1) of the bidiagonalization of a copy of the
+ * m x n device matrix dA (m >= n >= 2), with no other kernel of those column
+ * steps in between, for hardware-counter passes over that kernel alone.  The
+ * launches read and write only library workspace; results are discarded. */
+int svd_gpu_profile_left_gemv(int m, int n, const double *dA, int lda, int steps,
+                              void *stream);
+
 /* Human-readable description of the last failure (empty if none). */
 const char *svd_gpu_last_error(void);
 

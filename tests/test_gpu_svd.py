@@ -363,3 +363,22 @@ def test_form_u_v_matches_oracle(ddcsvd, po, shape):
     B[np.arange(n), np.arange(n)] = al
     B[np.arange(n - 1), np.arange(1, n)] = be
     assert np.linalg.norm(U.T @ A @ V - B) <= 1e-13 * max(1.0, np.linalg.norm(A)) * max(m, 8) ** 0.5
+
+
+def test_profile_left_gemv_hook(ddcsvd):
+    """The counter-pass hook runs the left-GEMV launches of every column step
+    on a workspace copy (the caller's matrix is unchanged) and rejects step
+    counts outside 1..n-1 and wide shapes."""
+    import torch
+    A = gen.normal_matrix(700, 300, 35)
+    At = ddcsvd.col_major(torch.from_numpy(A).to("cuda"))
+    gpu_case("profile_left_gemv 700x300, 299 steps")
+    ddcsvd.profile_left_gemv(At, 299)
+    torch.cuda.synchronize()
+    assert np.array_equal(At.cpu().numpy(), A)
+    for steps in (0, 300):
+        with pytest.raises(ddcsvd.SvdGpuError):
+            ddcsvd.profile_left_gemv(At, steps)
+    with pytest.raises(ddcsvd.SvdGpuError):
+        ddcsvd.profile_left_gemv(ddcsvd.col_major(torch.zeros((3, 5), dtype=torch.float64,
+                                                              device="cuda")), 1)
