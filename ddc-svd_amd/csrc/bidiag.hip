@@ -576,9 +576,8 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   }
   const double ut = (double)Ur[(int64_t)t * a.lda];
   double c = next ? (double)a.A[rc + (int64_t)(k + 1) * a.lda] : 0.0;
-  double rn = 0.0;
-  if (!noright)
-    for (int i = tid; i < a.nrn; i += FIN) rn += a.rnorm[i];
+  const int nrn = noright ? 0 : a.nrn;
+  const double rn0 = a.rnorm[min(tid, max(nrn - 1, 0))];  // used if tid < nrn
   const double rk1 = a.rvec[k + 1];
   // this thread's column of the right reflector (written below)
   const int jw = k + 1 + blockIdx.x * FIN + tid;
@@ -587,8 +586,45 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   const int td = dotr ? tid : 0;
   const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
                               : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
-  const double ds = noright ? 0.0 : sum_strided(a.Rdot + td, nccR, DSTRIDE);
-  const double h = noright ? 0.0 : sum_strided(a.Rpart + rc, nccR, a.lda);  // A0[row, k+1:n] r
+  // the first PB chunk partials of the panel dots (Rdot) and of this row's
+  // A0 r (Rpart), clamped to the last chunk, in the same batch as the loads
+  // above: one wait for all of them
+  constexpr int PB = 32;
+  const int ncl = noright ? 1 : nccR;
+  double pd[PB], ph[PB];
+#pragma unroll
+  for (int u = 0; u < PB; ++u) {
+    const int q = min(u, ncl - 1);
+    pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
+    ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // sums in chunk order (as a sequential sum over the chunks)
+  double rn = 0.0, ds = 0.0, h = 0.0;  // h: A0[row, k+1:n] r
+  if (!noright) {
+    if (tid < nrn) rn += rn0;
+    for (int i = tid + FIN; i < nrn; i += FIN) rn += a.rnorm[i];
+#pragma unroll
+    for (int u = 0; u < PB; ++u)
+      if (u < nccR) {
+        ds += pd[u];
+        h += ph[u];
+      }
+    for (int q = PB; q < nccR; q += PB) {
+#pragma unroll
+      for (int u = 0; u < PB; ++u) {
+        const int qq = min(q + u, nccR - 1);
+        pd[u] = a.Rdot[td + (int64_t)qq * DSTRIDE];
+        ph[u] = a.Rpart[rc + (int64_t)qq * a.lda];
+      }
+#pragma unroll
+      for (int u = 0; u < PB; ++u)
+        if (q + u < nccR) {
+          ds += pd[u];
+          h += ph[u];
+        }
+    }
+  }
 
   Refl R;
   R.nrm = R.sgn = R.rl = 0.0;
