@@ -227,7 +227,8 @@ def test_bdsvd_vectors_vs_golden_twisted(ddcsvd):
 # ---------------------------------------------------------------- full svd_gpu
 
 SVD_SHAPES = [(1, 1), (5, 1), (1, 5), (2, 2), (3, 3), (7, 5), (5, 7), (33, 31), (64, 64),
-              (96, 64), (64, 96), (130, 100), (300, 300), (512, 256), (4200, 300), (300, 4200)]
+              (96, 64), (64, 96), (130, 100), (300, 300), (512, 256), (768, 500), (256, 768),
+              (4200, 300), (300, 4200)]
 
 
 @pytest.mark.parametrize("shape", SVD_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
