@@ -327,7 +327,7 @@ static int left_gemv_dev(int m, int n, const double* dA, int lda, int steps, hip
   }
   DDC_TRY(ctx_init());
   hipStream_t st = user ? user : g_ctx.st;
-  const int lda_p = (int)round_up(m, 16);
+  const int lda_p = tall_lda(m);  // the layout of the SVD's working copy
   ArenaPlan plan;
   plan.add<double>((size_t)lda_p * n);
   plan.add<double>(n);
