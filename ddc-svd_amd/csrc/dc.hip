@@ -263,7 +263,8 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
       }
       prev = j;
     }
-    if (fabs(wz[0]) <= tol) wz[0] = tol;
+    // a zero arrow (every pole and z at zero) keeps its root at zero
+    if (fabs(wz[0]) <= tol) wz[0] = s_scale > 0.0 ? tol : 0.0;
   }
   __syncthreads();
 

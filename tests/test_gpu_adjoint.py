@@ -114,6 +114,9 @@ def test_gap_guard_repeated_singular_values(ddcsvd):
     gU, gs, gV = _cotangents(m, n, 2)
     s, U, V, gA = _gpu_grad(ddcsvd, A, gU, gs, gV, guard=1e-10)
     assert np.all(np.isfinite(gA))
+    # the forward factors stay orthonormal through the exact repeat
+    assert np.linalg.norm(U.T @ U - np.eye(n)) <= 2e-13 * n
+    assert np.linalg.norm(V.T @ V - np.eye(n)) <= 2e-13 * n
     # d / delta^2 stays below 1 / delta in size, delta = guard * s_max^2
     bound = (1.0 / (1e-10 * s[-1] ** 2)) * (np.linalg.norm(gU) + np.linalg.norm(gV)) * 4
     assert np.linalg.norm(gA) <= bound

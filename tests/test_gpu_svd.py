@@ -131,6 +131,8 @@ def test_bdsvd_values_closed_forms(ddcsvd):
     # 1 x 1: |a|; diagonal: sorted |a|; 2 x 2 [[a, b], [0, c]]
     gpu_case("bdsvd_values closed forms")
     assert np.allclose(ddcsvd.bdsvd_values([-3.0], []), [3.0], rtol=0, atol=0)
+    for k in (1, 2, 5, 40):  # B = 0
+        assert np.array_equal(ddcsvd.bdsvd_values(np.zeros(k), np.zeros(k - 1)), np.zeros(k))
     a = np.array([3.0, -1.0, 2.0, 0.5])
     s = ddcsvd.bdsvd_values(a, np.zeros(3))
     assert np.array_equal(s, np.sort(np.abs(a)))
@@ -336,6 +338,9 @@ def test_svd_gpu_zero_and_rank_deficient(ddcsvd):
     assert np.max(s[:-1]) <= 1e-13 * s[-1]
     res = np.linalg.norm(A - (U * s) @ V.T) / np.linalg.norm(A)
     assert res <= 1e-13
+    # the 29-fold zero cluster still gets 29 orthonormal vectors
+    assert np.linalg.norm(U.T @ U - np.eye(30)) <= 2e-13 * 30
+    assert np.linalg.norm(V.T @ V - np.eye(30)) <= 2e-13 * 30
 
 
 def test_svd_gpu_rejects_nonfinite(ddcsvd):
