@@ -836,6 +836,47 @@ static void bj_free(svd_gpu_batch* p) {
   p->d_work = nullptr;
 }
 
+// Items (element offsets into the packed buffers and into the work area)
+// and costs of a batch, in batch order; off* (count + 1 entries) may be
+// null.  wa / wv: elements of the working copies of A and V.
+static void bj_layout(int count, const int* m, const int* n, BatchItem* items, double* cost,
+                      long long* offA, long long* offS, long long* offU, long long* offV,
+                      int64_t& wa, int64_t& wv) {
+  long long oa = 0, os = 0, ou = 0, ov = 0;
+  wa = wv = 0;
+  for (int k = 0; k < count; ++k) {
+    const int mk = m[k], nk = n[k], mn = std::min(mk, nk), mt = std::max(mk, nk);
+    if (offA) {
+      offA[k] = oa;
+      offS[k] = os;
+      offU[k] = ou;
+      offV[k] = ov;
+    }
+    BatchItem& it = items[k];
+    it.m = mk;
+    it.n = nk;
+    it.a = oa;
+    it.s = os;
+    it.u = ou;
+    it.v = ov;
+    it.wa = wa;
+    it.wv = wv;
+    oa += (long long)mk * nk;
+    os += mn;
+    ou += (long long)mk * mn;
+    ov += (long long)nk * mn;
+    wa += (int64_t)mt * mn;
+    wv += (int64_t)mn * mn;
+    cost[k] = (double)mn * mn * mt;
+  }
+  if (offA) {
+    offA[count] = oa;
+    offS[count] = os;
+    offU[count] = ou;
+    offV[count] = ov;
+  }
+}
+
 static int bj_plan(svd_gpu_batch* p, int count, const int* m, const int* n, int dtype) {
   const size_t es = dtype == SVD_GPU_F64 ? 8 : 4;
   p->count = count;
@@ -849,26 +890,9 @@ static int bj_plan(svd_gpu_batch* p, int count, const int* m, const int* n, int 
   std::vector<BatchItem> items(count);
   std::vector<double> cost(count);
   int64_t wa = 0, wv = 0;
-  for (int k = 0; k < count; ++k) {
-    const int mk = m[k], nk = n[k], mn = std::min(mk, nk), mt = std::max(mk, nk);
-    BatchItem& it = items[k];
-    it.m = mk;
-    it.n = nk;
-    it.a = p->offA[k];
-    it.s = p->offS[k];
-    it.u = p->offU[k];
-    it.v = p->offV[k];
-    it.wa = wa;
-    it.wv = wv;
-    p->offA[k + 1] = p->offA[k] + (long long)mk * nk;
-    p->offS[k + 1] = p->offS[k] + mn;
-    p->offU[k + 1] = p->offU[k] + (long long)mk * mn;
-    p->offV[k + 1] = p->offV[k] + (long long)nk * mn;
-    wa += (int64_t)mt * mn;
-    wv += (int64_t)mn * mn;
-    cost[k] = (double)mn * mn * mt;
-    p->work += cost[k];
-  }
+  bj_layout(count, m, n, items.data(), cost.data(), p->offA.data(), p->offS.data(),
+            p->offU.data(), p->offV.data(), wa, wv);
+  for (int k = 0; k < count; ++k) p->work += cost[k];
   p->work_elems_a = (size_t)round_up(wa, 64);
   p->work_elems_v = (size_t)round_up(wv, 64);
   p->order.resize(count);
@@ -930,6 +954,117 @@ static int bj_run(svd_gpu_batch* p, const void* dA, void* dS, void* dU, void* dV
   }
   DDC_HIP(hipGraphLaunch(p->exec, st));
   return DDC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The variant that allocates nothing: every buffer beyond the caller's inputs
+// and outputs (matrix descriptors, processing order, queue head, working
+// copies) lives in one file-scope device array, and the host-side
+// descriptors in file-scope host arrays.  Each GPU holds its own copy of the
+// device array (its address is looked up once per device); calls on one GPU
+// share it, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t BJ_NA_BYTES = (size_t)32 << 20;  // device scratch per GPU
+constexpr int BJ_NA_MAXCOUNT = 4096;              // matrices per call
+constexpr int BJ_NA_MAXDEV = 64;
+// fixed launch configuration: variant 1 (column blocks of 4), two resident
+// workgroups per CU, whole-workgroup factorization from min(m, n) = 32 on
+static const BjConfig kNoallocConfig = {1, 2, 32};
+
+static __device__ __align__(256) char scratch[BJ_NA_BYTES];
+
+namespace {
+std::mutex g_na_mu;
+char* g_na_ptr[BJ_NA_MAXDEV] = {};
+int g_na_cus[BJ_NA_MAXDEV] = {};
+BatchItem g_na_items[BJ_NA_MAXCOUNT];
+double g_na_cost[BJ_NA_MAXCOUNT];
+int g_na_order[BJ_NA_MAXCOUNT];
+}  // namespace
+
+static size_t na_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int bj_noalloc(int count, const int* m, const int* n, int dtype, const void* dA, void* dS,
+                      void* dU, void* dV, int* info, hipStream_t st) {
+  int dev = 0;
+  DDC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= BJ_NA_MAXDEV) {
+    report_error("device index beyond the no-alloc table");
+    return DDC_E_ARG;
+  }
+  if (!g_na_ptr[dev]) {
+    // set-up, once per device with that device current: the device address
+    // of this GPU's copy of the scratch array (the array's name in host
+    // code is only a host-side handle)
+    void* ptr = nullptr;
+    DDC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    int cus = 0;
+    DDC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    g_na_cus[dev] = cus > 0 ? cus : 1;
+    g_na_ptr[dev] = (char*)ptr;
+  }
+  char* base = g_na_ptr[dev];
+  const size_t es = dtype == SVD_GPU_F64 ? 8 : 4;
+  int64_t wa = 0, wv = 0;
+  bj_layout(count, m, n, g_na_items, g_na_cost, nullptr, nullptr, nullptr, nullptr, wa, wv);
+  const size_t b_items = na_align(sizeof(BatchItem) * count);
+  const size_t b_order = na_align(sizeof(int) * count);
+  const size_t b_ctr = 256;
+  const size_t ea = (size_t)round_up(wa, 64), ev = (size_t)round_up(wv, 64);
+  const size_t need = b_items + b_order + b_ctr + na_align(es * (ea + ev));
+  if (need > BJ_NA_BYTES) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "batch needs %zu bytes of scratch; the no-alloc limit is %zu",
+             need, BJ_NA_BYTES);
+    report_error(msg);
+    return DDC_E_ARG;
+  }
+  for (int k = 0; k < count; ++k) g_na_order[k] = k;
+  // largest first, ties by index (std::sort works in place)
+  std::sort(g_na_order, g_na_order + count, [](int x, int y) {
+    return g_na_cost[x] > g_na_cost[y] || (g_na_cost[x] == g_na_cost[y] && x < y);
+  });
+  BatchItem* d_items = (BatchItem*)base;
+  int* d_order = (int*)(base + b_items);
+  int* d_ctr = (int*)(base + b_items + b_order);
+  char* d_work = base + b_items + b_order + b_ctr;
+  DDC_HIP(hipMemcpyAsync(d_items, g_na_items, sizeof(BatchItem) * count, hipMemcpyHostToDevice,
+                         st));
+  DDC_HIP(hipMemcpyAsync(d_order, g_na_order, sizeof(int) * count, hipMemcpyHostToDevice, st));
+  // the host arrays are reused by the next call
+  DDC_HIP(hipStreamSynchronize(st));
+  // the queue head starts at zero (a fill kernel; the array keeps the last
+  // call's contents)
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, d_ctr, 0, 1);
+  DDC_LAUNCHED();
+  if (dtype == SVD_GPU_F64) {
+    BjArgs<double> a;
+    a.count = count;
+    a.items = d_items;
+    a.order = d_order;
+    a.counter = d_ctr;
+    a.A = (const double*)dA;
+    a.S = (double*)dS;
+    a.U = (double*)dU;
+    a.V = (double*)dV;
+    a.W = (double*)d_work;
+    a.WV = (double*)d_work + ea;
+    a.info = info;
+    return bj_launch<double>(kNoallocConfig, g_na_cus[dev], a, st);
+  }
+  BjArgs<float> a;
+  a.count = count;
+  a.items = d_items;
+  a.order = d_order;
+  a.counter = d_ctr;
+  a.A = (const float*)dA;
+  a.S = (float*)dS;
+  a.U = (float*)dU;
+  a.V = (float*)dV;
+  a.W = (float*)d_work;
+  a.WV = (float*)d_work + ea;
+  a.info = info;
+  return bj_launch<float>(kNoallocConfig, g_na_cus[dev], a, st);
 }
 
 }  // namespace ddc
@@ -1033,6 +1168,23 @@ int svd_gpu_batched_run(svd_gpu_batch* plan, const void* dA, void* dsigma, void*
   }
   std::lock_guard<std::mutex> lk(plan->mu);
   return bj_run(plan, dA, dsigma, dU, dV, dinfo, (hipStream_t)stream);
+}
+
+int svd_gpu_batched_noalloc(int count, const int* m, const int* n, int dtype, const void* dA,
+                            void* dsigma, void* dU, void* dV, int* dinfo, void* stream) {
+  clear_error();
+  if (count < 1 || count > BJ_NA_MAXCOUNT || !m || !n || !dA || !dsigma || !dU || !dV ||
+      (dtype != SVD_GPU_F64 && dtype != SVD_GPU_F32)) {
+    report_error("invalid batch arguments (1 <= count <= 4096 for the no-alloc entry point)");
+    return DDC_E_ARG;
+  }
+  for (int k = 0; k < count; ++k)
+    if (m[k] < 1 || n[k] < 1 || m[k] > BJ_MAXDIM || n[k] > BJ_MAXDIM) {
+      report_error("batched matrices need 1 <= m, n <= 256");
+      return DDC_E_ARG;
+    }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  return bj_noalloc(count, m, n, dtype, dA, dsigma, dU, dV, dinfo, (hipStream_t)stream);
 }
 
 void svd_gpu_batched_destroy(svd_gpu_batch* plan) {

@@ -40,6 +40,8 @@ def _bind(lib):
     lib.svd_gpu_batched_set_config.restype = i
     lib.svd_gpu_batched_destroy.argtypes = [v]
     lib.svd_gpu_batched_destroy.restype = None
+    lib.svd_gpu_batched_noalloc.argtypes = [i, pi, pi, i, v, v, v, v, v, v]
+    lib.svd_gpu_batched_noalloc.restype = i
     lib._batched_bound = True
     return lib
 
@@ -120,6 +122,18 @@ class BatchedSvd:
         ip = None if info is None else ctypes.c_void_p(info.data_ptr())
         _check(self.lib.svd_gpu_batched_run(self.plan, *self._ptrs(A, S, U, V), ip,
                                             ctypes.c_void_p(stream)))
+
+    def run_noalloc(self, A, S, U, V, info=None, stream=None):
+        """The same batch through svd_gpu_batched_noalloc (no plan buffers, no
+        device allocation; fixed launch configuration)."""
+        if stream is None:
+            stream = self.torch.cuda.current_stream().cuda_stream
+        ip = None if info is None else ctypes.c_void_p(info.data_ptr())
+        pi = ctypes.POINTER(ctypes.c_int)
+        _check(self.lib.svd_gpu_batched_noalloc(
+            self.count, self.ms.ctypes.data_as(pi), self.ns.ctypes.data_as(pi),
+            F64 if self.dtype == self.torch.float64 else F32, *self._ptrs(A, S, U, V), ip,
+            ctypes.c_void_p(stream)))
 
     def tune(self, A, S, U, V, stream=None):
         if stream is None:

@@ -184,9 +184,9 @@ int svd_gpu_batched_offsets(const svd_gpu_batch *plan, long long *offA,
  */
 int svd_gpu_batched_run(svd_gpu_batch *plan, const void *dA, void *dsigma,
                         void *dU, void *dV, int *dinfo, void *stream);
-/* Times the kernel variants, launch sizes and cooperation thresholds on half
- * of the batch (the result is also remembered for later plans of similar
- * batches); writes the outputs of that half.  Synchronous. */
+/* Times the kernel variants, launch sizes and cooperation thresholds on the
+ * whole batch (the result is also remembered for later plans of similar
+ * batches); writes the outputs.  Synchronous. */
 int svd_gpu_batched_tune(svd_gpu_batch *plan, const void *dA, void *dsigma,
                          void *dU, void *dV, void *stream);
 /* Chosen configuration: kernel variant, resident 4-wave workgroups per CU,
@@ -201,6 +201,25 @@ int svd_gpu_batched_config(const svd_gpu_batch *plan, int *variant,
 int svd_gpu_batched_set_config(svd_gpu_batch *plan, int variant, int wgs_per_cu,
                                int coop_min_n);
 void svd_gpu_batched_destroy(svd_gpu_batch *plan);
+
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the batched SVD without a plan, with the same packed
+ * buffer layout as svd_gpu_batched_offsets and the same kernel as
+ * svd_gpu_batched_run, at a fixed launch configuration (variant 1, two
+ * workgroups per CU, cooperation from min(m, n) = 32).  Its descriptors,
+ * queue head and working copies live in a 32 MiB device array that each GPU
+ * holds from the library's first kernel launch until the process ends; a
+ * batch needing more (about sum of max(m,n) min(m,n) + min(m,n)^2 elements,
+ * 8 or 4 bytes each), or more than 4096 matrices, is rejected with
+ * SVD_GPU_E_ARG.  Calls no hipMalloc/hipFree; copies the descriptors from
+ * host arrays and synchronizes the stream once before the launch.  Calls
+ * share the array: calls that overlap on one GPU race.
+ */
+int svd_gpu_batched_noalloc(int count, const int *m, const int *n, int dtype,
+                            const void *dA, void *dsigma, void *dU, void *dV,
+                            int *dinfo, void *stream);
 
 /* Milliseconds of the stages of the last successful svd_gpu / svd_gpu_status
  * / svd_gpu_dev call in this process: [0] copy-in, [1] bidiagonalization,
