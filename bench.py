@@ -10,23 +10,32 @@ point svd_gpu_dev, input already resident in HBM.  Default: 8192 x 8192.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--m 8192 --n 8192]
                     [--no-forms] [--no-cpu] [--no-check]
 
-Prints one JSON line.  `roofline` covers the dominant kernel, the left GEMV
-of the bidiagonalization (bd_left_gemv): its algorithmic traffic is one FP64
-read of the trailing block A[k:m, k+1:n] per launch, 8 (m-k)(n-k-1) bytes;
-achieved = sum of those bytes / sum of the launch durations, both over the
-timed steps, the durations from HIP events recorded around every launch on
-the library's stream (svd_gpu_profile_kernels).  `traffic` is the measured
-HBM bytes per launch from the committed rocprofv3 counter summary
-(profiles/bd_left_gemv_8192x8192_traffic.json) when it matches the workload.
-`cpu_baseline` times the serial bidiagonalization of the reference host path
-(the oracle restatement, bit-identical to the reference's bidiag_seq) on the
-first steps of the same matrix and scales by the algorithmic work ratio.
+Prints one JSON line.  `roofline` covers the dominant kernel of the SVD, the
+one with the largest total time per SVD among the bidiagonalization's
+GEMVs: the right GEMV (bd_right_gemv), whose algorithmic traffic is one FP64
+read of the trailing block A[k+1:m, k+1:n] per launch, 8 (m-k-1)(n-k-1)
+bytes; achieved = the sum of those bytes / the sum of the launch durations
+over the timed steps.  The durations are stamped by the hardware at the
+start and the end of each launch (hipExtLaunchKernel with start and stop
+events, on the library's stream; svd_gpu_profile_kernels), in extra steps
+kept out of `value`.  The left GEMV (bd_left_gemv, 8 (m-k)(n-k-1) bytes per
+launch) is reported beside it in `kernels`, with the counter-measured HBM
+traffic of profiles/bd_left_gemv_{m}x{n}_traffic.json.
+`cpu_baseline` is the reference's host path (the oracle restatement,
+bit-identical to the compiled reference) on the box's host cores: the
+serial bidiagonalization (one core by construction) timed on the first
+steps of the same matrix and scaled by the work ratio, plus the OpenMP
+stages (values, vectors, back-transformation) from full host-path runs of
+a 1024 x 1024 matrix of the same distribution (median of three), scaled by
+their orders of growth; the 1024^2 full path, extrapolated by the survey's
+x8 .. x11.7 per doubling, brackets the bidiagonalization estimate.
 `cpu_lapack` is an informative, stronger CPU figure: numpy's LAPACK SVD
 (U, sigma, V) on all host threads, timed on a half-size matrix of the same
 distribution and scaled by the work ratio S(m,n)/S(m/2,n/2) = 8.
 `forms` holds the batched SVD (10,000 matrices, m ~ U{16..256}, n ~ U{8..m},
 entries U(-1,1)) in FP64 and FP32 against a loop of plain svd_gpu_dev calls,
 the forward + backward throughput of the differentiable SVD at 2048 x 1024,
+the plain path on the second plain configuration (8192 x 4096, N(0,1)),
 and the mixed-precision SVD of an 8192^2 matrix with singular values graded
 from 1 to 1e-6 against the plain FP64 path (errors per decade of sigma), and
 of the plain workload's own N(0,1) matrix (residual, orthogonality, speed).
@@ -62,9 +71,22 @@ def S(m: int, n: int) -> int:
     return (M - N) * N * (N + 1) // 2 + N * (N + 1) * (2 * N + 1) // 6
 
 
-def cpu_baseline(m: int, n: int, seed: int, steps: int) -> dict:
-    """Serial bidiagonalization (1 core) of the first `steps` columns of the
-    same matrix, scaled to the whole reduction by the work ratio."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(m: int, n: int, seed: int, steps: int, small: int = 1024, reps: int = 3) -> dict:
+    """The reference's host path on the host cores: the serial
+    bidiagonalization of the first `steps` columns of the same matrix scaled
+    by the work ratio, plus the remaining (OpenMP) stages from full host-path
+    runs at small x small, scaled by their order of growth."""
     import ctypes
 
     import numpy as np
@@ -81,27 +103,47 @@ def cpu_baseline(m: int, n: int, seed: int, steps: int) -> dict:
     lib.orc_bidiag_sample(ctypes.c_int(m), ctypes.c_int(n), po._ptr(A), po._ptr(alpha),
                           po._ptr(beta), ctypes.c_int(steps))
     dt = time.perf_counter() - t0
+    del A
     sample = sum((m - i) * (n - i) for i in range(steps))
-    est = dt * S(m, n) / sample
-    cpu = "unknown"
-    try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        for line in out.splitlines():
-            if line.startswith("Model name"):
-                cpu = line.split(":", 1)[1].strip()
-    except OSError:
-        pass
+    t_bd = dt * S(m, n) / sample
+    # full host path (orc_svd: bidiagonalization, values, vectors,
+    # back-transformation) at small x small, median of `reps`
+    As = gen.normal_matrix(small, small, seed)
+    runs = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        po.oracle_svd(As)
+        runs.append((time.perf_counter() - t0, po.oracle_timings().copy()))
+    runs.sort(key=lambda r: r[0])
+    t_small, st = runs[len(runs) // 2]
+    mn = min(m, n)
+    g2 = (mn / small) ** 2                     # values and vectors: O(n^2) work
+    g3 = S(m, n) / S(small, small)             # back-transformation: O(n^3) work
+    t_rest = (st[1] + st[2]) * g2 + st[3] * g3
+    est = t_bd + t_rest
+    doublings = np.log2(mn / small)
+    lo, hi = st[0] * 8.0 ** doublings, st[0] * 11.7 ** doublings
+    threads = os.environ.get("OMP_NUM_THREADS") or str(os.cpu_count())
     return {
         "value": 1.0 / est,
         "unit": "SVD/s",
-        "cores": 1,
+        "cores": int(threads) if threads.isdigit() else threads,
         "kind": "port",
-        "sample": (f"serial bidiagonalization (oracle restatement of bidiag.c:33-186, "
-                   f"bit-identical to the reference) of the first {steps} of {min(m, n)} "
-                   f"steps of the same {m}x{n} N(0,1) seed-{seed} matrix: {dt:.1f} s, "
-                   f"scaled by S(m,n)/sample work = {S(m, n) / sample:.1f} to {est:.0f} s "
-                   f"per SVD; the remaining host stages (about 2% of the reference's "
-                   f"host time at 4096^2) are left out, which flatters the CPU; {cpu}"),
+        "sample": (f"the reference host path (oracle restatement, bit-identical to the compiled "
+                   f"reference) for one {m}x{n} N(0,1) seed-{seed} SVD, estimated at {est:.0f} s: "
+                   f"serial bidiagonalization (1 core, bidiag.c:33-186) {t_bd:.0f} s from its "
+                   f"first {steps} of {mn} steps on the same matrix ({dt:.1f} s, x{S(m, n) / sample:.1f} "
+                   f"work ratio); values + vectors {(st[1] + st[2]) * g2:.1f} s and "
+                   f"back-transformation {st[3] * g3:.1f} s ({threads} OpenMP threads) from the "
+                   f"full host path at {small}x{small} (median of {reps}: {t_small:.2f} s, stages "
+                   f"{st[0]:.2f} / {st[1]:.3f} / {st[2]:.3f} / {st[3]:.3f} s) scaled by x{g2:.0f} "
+                   f"(O(n^2)) and x{g3:.0f} (O(n^3)); the {small}^2 bidiagonalization extrapolated "
+                   f"by x8 .. x11.7 per doubling brackets the first term: {lo:.0f} .. {hi:.0f} s; "
+                   f"{cpu_model()}"),
+        "bidiag_estimate_s": t_bd,
+        "other_stages_estimate_s": t_rest,
+        "full_path_small_s": t_small,
+        "bidiag_bracket_from_small_s": [lo, hi],
     }
 
 
@@ -141,13 +183,49 @@ def right_gemv_bytes(m: int, n: int) -> int:
     return sum(8 * (m - k - 1) * (n - k - 1) for k in range(n - 2))
 
 
-def traffic_summary(m: int, n: int):
-    path = os.path.join(ROOT, "profiles", f"bd_left_gemv_{m}x{n}_traffic.json")
+def traffic_summary(m: int, n: int, kernel: str = "bd_left_gemv"):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{m}x{n}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
     return t.get("hbm_bytes_per_launch")
+
+
+def plain_form(m: int, n: int, seed: int = 2, steps: int = 3) -> dict:
+    """The plain path on another shape of the plain configuration: time per
+    SVD (device-resident input), stage times, bidiagonalization GB/s,
+    residual and orthogonality."""
+    import torch
+
+    import ddcsvd
+    from ddcsvd import gen
+
+    A = ddcsvd.col_major(torch.from_numpy(gen.normal_matrix(m, n, seed)).cuda())
+    mn = min(m, n)
+    sigma = torch.empty(mn, dtype=torch.float64, device="cuda")
+    U = ddcsvd.col_major(torch.empty((m, mn), dtype=torch.float64, device="cuda"))
+    V = ddcsvd.col_major(torch.empty((n, mn), dtype=torch.float64, device="cuda"))
+    stream = torch.cuda.current_stream().cuda_stream
+    ddcsvd.svd_dev(A, sigma, U, V, stream)
+    torch.cuda.synchronize()
+    stage = {k: 0.0 for k in ddcsvd.TIMING_KEYS}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ddcsvd.svd_dev(A, sigma, U, V, stream)
+        for k, v in ddcsvd.last_timings().items():
+            stage[k] += v / steps
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    eye = torch.eye(mn, dtype=torch.float64, device="cuda")
+    res = float(torch.linalg.norm(A - (U * sigma) @ V.T) / torch.linalg.norm(A))
+    ou = float(torch.linalg.norm(U.T @ U - eye))
+    ov = float(torch.linalg.norm(V.T @ V - eye))
+    return {"config": f"plain FP64 svd_gpu {m}x{n}, N(0,1) entries, seed {seed}, "
+                      f"device-resident input", "ms": 1e3 * sec, "svd_per_s": 1.0 / sec,
+            "steps": steps, "stages_ms": {k: stage[k] for k in ddcsvd.TIMING_KEYS[:6]},
+            "bidiag_GBs": 16.0 * S(m, n) / (stage["bidiag"] / 1e3) / 1e9,
+            "residual": res, "orth_u": ou, "orth_v": ov}
 
 
 def batched_form(seed: int = 0, count: int = 10000, reps: int = 3, loop_sample: int = 100,
@@ -529,11 +607,19 @@ def main() -> int:
     progress(f"kernel-timed {args.steps} steps: {ms_profiled:.1f} ms per SVD")
 
     mt, nt = max(m, n), mn
-    bl, br = left_gemv_bytes(mt, nt), right_gemv_bytes(mt, nt)
-    tl, tr = kt["left_gemv_ms"] / 1e3, kt["right_gemv_ms"] / 1e3
-    nl = max(kt["left_gemv_launches"], 1.0)
-    achieved = bl / tl / 1e9 if tl > 0 else None
-    traffic = traffic_summary(m, n)
+    gemv = {}
+    for name, nbytes, key in (("bd_left_gemv", left_gemv_bytes(mt, nt), "left_gemv"),
+                              ("bd_right_gemv", right_gemv_bytes(mt, nt), "right_gemv")):
+        t = kt[f"{key}_ms"] / 1e3
+        nl = max(kt[f"{key}_launches"], 1.0)
+        gemv[name] = {"achieved_GBs": nbytes / t / 1e9 if t > 0 else None,
+                      "algorithmic_bytes_per_launch": nbytes / nl,
+                      "avg_launch_us": 1e6 * t / nl if t > 0 else None,
+                      "launches_per_svd": nl, "ms_per_svd": 1e3 * t,
+                      "traffic": traffic_summary(m, n, name)}
+    dom = max(gemv, key=lambda k: gemv[k]["ms_per_svd"])
+    g = gemv[dom]
+    achieved = g["achieved_GBs"]
     t_bd = stage["bidiag"] / 1000.0
     out = {
         "metric": METRIC,
@@ -554,20 +640,18 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic,
-                     "kernel": "bd_left_gemv",
-                     "algorithmic_bytes_per_launch": bl / nl,
-                     "avg_launch_us": 1e6 * tl / nl if tl > 0 else None,
-                     "launches_per_svd": nl,
-                     "timing": f"HIP events around every launch over {args.steps} extra "
-                               f"steps ({ms_profiled:.0f} ms per SVD with the events)"},
-        "kernels": {
-            "bd_right_gemv": {"achieved_GBs": br / tr / 1e9 if tr > 0 else None,
-                              "avg_launch_us": 1e6 * tr / max(kt["right_gemv_launches"], 1),
-                              "algorithmic_bytes_per_svd": br},
-            "bidiag_stage": {"achieved_GBs": 16.0 * S(m, n) / t_bd / 1e9,
-                             "algorithmic_bytes_per_svd": 16 * S(m, n),
-                             "note": "two FP64 reads of the trailing matrix per step"}},
+                     "traffic": g["traffic"],
+                     "kernel": dom,
+                     "algorithmic_bytes_per_launch": g["algorithmic_bytes_per_launch"],
+                     "avg_launch_us": g["avg_launch_us"],
+                     "launches_per_svd": g["launches_per_svd"],
+                     "timing": f"start/stop stamps of every launch (hipExtLaunchKernel events) "
+                               f"over {args.steps} extra steps ({ms_profiled:.0f} ms per SVD "
+                               f"in those steps)"},
+        "kernels": dict(gemv, bidiag_stage={
+            "achieved_GBs": 16.0 * S(m, n) / t_bd / 1e9,
+            "algorithmic_bytes_per_svd": 16 * S(m, n),
+            "note": "two FP64 reads of the trailing matrix per step"}),
         "stages_ms": {k: stage[k] for k in ddcsvd.TIMING_KEYS[:6]},
     }
     if not args.no_check:
@@ -594,7 +678,9 @@ def main() -> int:
     progress(f"host-buffer entry point: {host_ms:.1f} ms")
     progress("checks done")
     if not args.no_forms:
-        out["forms"] = {"batched": batched_form()}
+        out["forms"] = {"plain_8192x4096": plain_form(8192, 4096)}
+        progress("plain 8192x4096 done")
+        out["forms"]["batched"] = batched_form()
         out["forms"]["backward"] = backward_form()
         progress("backward form done")
         out["forms"]["mixed"] = mixed_form()

@@ -860,11 +860,14 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     const int ncb = (int)ceil_div(n - k - 1, CBL);
     const int ndl = (int)ceil_div(2 * a.t, DPW);
     const int nchL = (int)ceil_div(m - rb0, RC);
-    const int kl0 = kclock_on() ? kclock_mark(st) : -1;
-    hipLaunchKernelGGL((bd_left_gemv<TS, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0, st, a,
-                       ncb, rb0);
+    hipEvent_t ke0, ke1;
+    if (kclock_on() && kclock_pair(KCLOCK_LEFT_GEMV, &ke0, &ke1) == 0)
+      hipExtLaunchKernelGGL((bd_left_gemv<TS, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0,
+                            st, ke0, ke1, 0, a, ncb, rb0);
+    else
+      hipLaunchKernelGGL((bd_left_gemv<TS, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0, st,
+                         a, ncb, rb0);
     DDC_LAUNCHED();
-    if (kl0 >= 0) kclock_span(KCLOCK_LEFT_GEMV, kl0, kclock_mark(st));
     // row k of M (the left finalization), the right reflector of row k and
     // the right GEMV
     const int nfr = (int)ceil_div(m - k - 1, FIN);
@@ -879,11 +882,13 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       const int rbr = ((k + 1) / 64) * 64;
       const int nrb = (int)ceil_div(m - rbr, RBR);
       const int ncc = (int)ceil_div(n - k - 1, CCR);
-      const int kr0 = kclock_on() ? kclock_mark(st) : -1;
-      hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st, a,
-                         nrb, rbr, nchL);
+      if (kclock_on() && kclock_pair(KCLOCK_RIGHT_GEMV, &ke0, &ke1) == 0)
+        hipExtLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st,
+                              ke0, ke1, 0, a, nrb, rbr, nchL);
+      else
+        hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st, a,
+                           nrb, rbr, nchL);
       DDC_LAUNCHED();
-      if (kr0 >= 0) kclock_span(KCLOCK_RIGHT_GEMV, kr0, kclock_mark(st));
       a.nrn = ncc;
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();

@@ -66,6 +66,25 @@ int kclock_mark(hipStream_t st) {
   return g_kc.used++;
 }
 
+int kclock_pair(int cls, hipEvent_t* e0, hipEvent_t* e1) {
+  for (int k = 0; k < 2; ++k) {
+    if (g_kc.used + k == (int)g_kc.pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        g_kc.failed = true;
+        return -1;
+      }
+      g_kc.pool.push_back(e);
+    }
+  }
+  const int a = g_kc.used, b = g_kc.used + 1;
+  g_kc.used += 2;
+  *e0 = g_kc.pool[a];
+  *e1 = g_kc.pool[b];
+  kclock_span(cls, a, b);
+  return 0;
+}
+
 void kclock_span(int cls, int a, int b) {
   if (a >= 0 && b >= 0 && cls >= 0 && cls < KCLOCK_CLASSES) g_kc.spans[cls].emplace_back(a, b);
 }
@@ -119,7 +138,7 @@ static int ctx_init() {
   return DDC_OK;
 }
 
-static int arena_get(size_t bytes, Arena& ws) {
+static int arena_get(size_t bytes, Arena& ws, hipStream_t st) {
   if (bytes > g_ctx.cap) {
     if (g_ctx.arena) DDC_HIP(hipFree(g_ctx.arena));
     g_ctx.arena = nullptr;
@@ -134,6 +153,11 @@ static int arena_get(size_t bytes, Arena& ws) {
   ws.base = g_ctx.arena;
   ws.cap = g_ctx.cap;
   ws.used = 0;
+  // test aid: with DDCSVD_POISON_ARENA set, every call starts from a
+  // workspace of NaN bytes, so a buffer read before this call writes it
+  // shows up in the results
+  static const bool poison = getenv("DDCSVD_POISON_ARENA") != nullptr;
+  if (poison) DDC_HIP(hipMemsetAsync(g_ctx.arena, 0xff, g_ctx.cap, st));
   return DDC_OK;
 }
 
@@ -281,7 +305,7 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
   tall_plan(plan, mt, nt, true);
   if (wide) plan.add<double>((size_t)m * n);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   TallBufs b;
   DDC_TRY(tall_take(ws, mt, nt, true, b));
   double* Araw = wide ? ws.take<double>((size_t)m * n) : nullptr;
@@ -326,14 +350,14 @@ static int left_gemv_dev(int m, int n, const double* dA, int lda, int steps, hip
     return DDC_E_ARG;
   }
   DDC_TRY(ctx_init());
-  hipStream_t st = user ? user : g_ctx.st;
+  hipStream_t st = user;  // NULL: the null stream, ordered with the caller's other work
   const int lda_p = tall_lda(m);  // the layout of the SVD's working copy
   ArenaPlan plan;
   plan.add<double>((size_t)lda_p * n);
   plan.add<double>(n);
   bidiag_plan(plan, m, n, lda_p);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   double* Ap = ws.take<double>((size_t)lda_p * n);
   double* da = ws.take<double>(n);
   DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
@@ -353,13 +377,13 @@ static int svd_dev(int m, int n, const double* dA, int lda, double* dsigma, doub
     return DDC_E_ARG;
   }
   DDC_TRY(ctx_init());
-  hipStream_t st = user ? user : g_ctx.st;
+  hipStream_t st = user;  // NULL: the null stream, ordered with the caller's other work
   const bool wide = m < n;
   const int mt = wide ? n : m, nt = mn;
   ArenaPlan plan;
   tall_plan(plan, mt, nt, wide);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   TallBufs b;
   DDC_TRY(tall_take(ws, mt, nt, wide, b));
   Timer tm;
@@ -405,7 +429,7 @@ static int bidiag_host(int m, int n, double* A, double* alpha, double* beta) {
   plan.add<double>(n);
   bidiag_plan(plan, m, n, lda_p);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   double* Ap = ws.take<double>((size_t)lda_p * n);
   double* da = ws.take<double>(n);
   double* db = ws.take<double>(n);
@@ -443,7 +467,7 @@ static int bdsvd_host(int n, const double* alpha, const double* beta, double* si
     twisted_plan(plan, n);
   }
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   double* da = ws.take<double>(n);
   double* db = ws.take<double>(n);
   double* ds = ws.take<double>(n);
@@ -482,14 +506,14 @@ static int backward_dev(int m, int n, const double* U, int ldu, const double* si
     return DDC_E_ARG;
   }
   DDC_TRY(ctx_init());
-  hipStream_t st = user ? user : g_ctx.st;
+  hipStream_t st = user;  // NULL: the null stream, ordered with the caller's other work
   const bool wide = m < n;
   const int mt = wide ? n : m;
   ArenaPlan plan;
   adjoint_plan(plan, mt, mn);
   if (wide) plan.add<double>((size_t)mt * mn);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   if (!wide) {
     DDC_TRY(adjoint_run(m, n, U, ldu, sigma, V, ldv, gU, ldgu, gs, gV, ldgv, gA, ldga, guard,
                         ws, st));
@@ -525,7 +549,7 @@ static int form_host(bool left, int m, int n, const double* A, double* Q) {
   plan.add<double>((size_t)k * k);
   form_q_plan(plan, m, n, left);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   double* Ap = ws.take<double>((size_t)lda_p * n);
   double* Qd = ws.take<double>((size_t)k * k);
   if (!Qd) return DDC_E_NOMEM;
@@ -550,12 +574,12 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
     return DDC_E_ARG;
   }
   DDC_TRY(ctx_init());
-  hipStream_t st = user ? user : g_ctx.st;
+  hipStream_t st = user;  // NULL: the null stream, ordered with the caller's other work
   ArenaPlan plan;
   plan.add<int>(64);
   mixed_plan(plan, n, upd);
   Arena ws;
-  DDC_TRY(arena_get(plan.bytes, ws));
+  DDC_TRY(arena_get(plan.bytes, ws, st));
   int* flag = ws.take<int>(64);
   if (!flag) return DDC_E_NOMEM;
   DDC_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));

@@ -6,6 +6,7 @@
 // (cl-helper.h:47-55, svd_gpu.c:85-96); svd_gpu_status() returns the code.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -28,9 +29,12 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // unless svd_gpu_profile_kernels(1) was called).  kclock_on() tells whether
 // it is active; kclock_mark() records an event on st and returns its index;
 // kclock_span(cls, a, b) books the span between two marks to class cls.
+// kclock_pair() hands out two events for hipExtLaunchKernel, which stamps
+// them with the start and the end of that one kernel, and books the span.
 bool kclock_on();
 int kclock_mark(hipStream_t st);
 void kclock_span(int cls, int a, int b);
+int kclock_pair(int cls, hipEvent_t* e0, hipEvent_t* e1);
 enum { KCLOCK_LEFT_GEMV = 0, KCLOCK_RIGHT_GEMV = 1, KCLOCK_CLASSES = 2 };
 
 // Message returned by svd_gpu_last_error(); clear_error() empties it.

@@ -51,8 +51,9 @@ int svd_gpu_status(int m, int n, double *A, double *sigma, double *U,
  *   dsigma min(m,n) device values, ascending
  *   dU     m x min(m,n) device matrix, leading dimension ldu >= m
  *   dV     n x min(m,n) device matrix, leading dimension ldv >= n
- *   stream a hipStream_t (NULL = the library's own stream); the call is
- *          ordered on it and returns after the results are complete.
+ *   stream a hipStream_t (NULL = the null stream, so the call is also
+ *          ordered after the caller's work on blocking streams); the call
+ *          is ordered on it and returns after the results are complete.
  */
 int svd_gpu_dev(int m, int n, const double *dA, int lda, double *dsigma,
                 double *dU, int ldu, double *dV, int ldv, void *stream);

@@ -388,3 +388,24 @@ def test_profile_left_gemv_hook(ddcsvd):
     with pytest.raises(ddcsvd.SvdGpuError):
         ddcsvd.profile_left_gemv(ddcsvd.col_major(torch.zeros((3, 5), dtype=torch.float64,
                                                               device="cuda")), 1)
+
+
+def test_svd_dev_null_stream_orders_after_torch_work(ddcsvd):
+    """svd_gpu_dev with stream NULL (torch's default stream handle is 0) must
+    run after the caller's pending work on that stream: A is produced by a
+    chain of GEMMs still in flight when the call is made (the workspace is
+    already allocated, so no allocation synchronizes the device)."""
+    import torch
+    m, n = 1024, 512
+    A0 = gen.normal_matrix(m, n, 41)
+    ddcsvd.svd_torch(torch.from_numpy(A0).cuda())  # warm the workspace at this size
+    torch.cuda.synchronize()
+    X = torch.from_numpy(gen.normal_matrix(4096, 4096, 42)).cuda() / 64.0
+    gpu_case("svd_gpu_dev right after in-flight GEMMs on the null stream")
+    Y = X
+    for _ in range(24):
+        Y = Y @ X                       # ~0.1 s of queued work
+    A = ddcsvd.col_major(torch.from_numpy(A0).cuda() + 0.0 * Y[:m, :n])
+    s, U, V = ddcsvd.svd_torch(A)
+    torch.cuda.synchronize()
+    check_svd(A0, s.cpu().numpy(), U.cpu().numpy(), V.cpu().numpy(), vec=False)
