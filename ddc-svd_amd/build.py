@@ -33,7 +33,7 @@ FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["blas.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
+SOURCES = ["blas.hip", "gemm.hip", "bidiag.hip", "dc.hip", "twisted.hip", "backtrans.hip",
            "adjoint.hip", "batched.hip", "mixed.hip", "capi.hip"]
 
 

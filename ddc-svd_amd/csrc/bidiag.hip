@@ -899,8 +899,8 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       const int r0 = a.p + NB;
       TS* C = A + r0 + (int64_t)r0 * lda;
       if constexpr (sizeof(TS) == 8) {
-        DDC_TRY(dgemm('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0, d.ldy,
-                      1.0, C, lda, st));
+        DDC_TRY(dgemm_mfma('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0,
+                           d.ldy, 1.0, C, lda, st));
       } else {
         const float* Up = A + r0 + (int64_t)a.p * lda;
         const int ra = m - r0, rb = n - r0;

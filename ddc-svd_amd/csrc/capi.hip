@@ -703,6 +703,24 @@ int svd_gpu_profile_kernels(int on) {
   return DDC_OK;
 }
 
+int svd_gpu_dgemm(char transa, char transb, int M, int N, int K, double alpha, const double* dA,
+                  int lda, const double* dB, int ldb, double beta, double* dC, int ldc,
+                  void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  const bool ta = transa == 'T' || transa == 't', tb = transb == 'T' || transb == 't';
+  if (M < 0 || N < 0 || K < 0 || (!ta && transa != 'N' && transa != 'n') ||
+      (!tb && transb != 'N' && transb != 'n') || lda < (ta ? K : M) || lda < 1 ||
+      ldb < (tb ? N : K) || ldb < 1 || ldc < M || ldc < 1 || (M > 0 && N > 0 && !dC) ||
+      (K > 0 && (!dA || !dB))) {
+    set_err("invalid svd_gpu_dgemm arguments");
+    return DDC_E_ARG;
+  }
+  DDC_TRY(dgemm_mfma(transa, transb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc,
+                     (hipStream_t)stream));
+  return DDC_OK;
+}
+
 const char* svd_gpu_last_error(void) { return g_err; }
 
 }  // extern "C"

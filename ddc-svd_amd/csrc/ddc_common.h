@@ -172,6 +172,10 @@ int sgemm(char ta, char tb, int M, int N, int K, float alpha, const float* A, in
 // BF16 operands (raw 16-bit storage), FP32 accumulation and C.
 int gemm_bf16_f32(char ta, char tb, int M, int N, int K, float alpha, const void* A, int lda,
                   const void* B, int ldb, float beta, float* C, int ldc, hipStream_t st);
+// The library's own FP64 GEMM on the matrix cores (gemm.hip), same
+// arguments as dgemm.
+int dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, int lda,
+               const double* B, int ldb, double beta, double* C, int ldc, hipStream_t st);
 // B <- T^{-1} B for the M x M upper triangular T (non-unit diagonal).
 int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
                      hipStream_t st);

@@ -85,6 +85,10 @@ def load():
         lib.svd_gpu_profile_left_gemv.restype = i
         lib.svd_gpu_last_error.argtypes = []
         lib.svd_gpu_last_error.restype = ctypes.c_char_p
+        c = ctypes.c_char
+        lib.svd_gpu_dgemm.argtypes = [c, c, i, i, i, ctypes.c_double, v, i, v, i,
+                                      ctypes.c_double, v, i, v]
+        lib.svd_gpu_dgemm.restype = i
         _lib = lib
     return _lib
 
@@ -178,7 +182,11 @@ def svd_torch(A):
 
 
 def _colmajor_ld(t, name: str) -> int:
-    if t.dtype != _torch().float64 or not t.is_cuda or t.dim() != 2 or t.stride(0) != 1:
+    if t.dtype != _torch().float64 or not t.is_cuda or t.dim() != 2:
+        raise ValueError(f"{name} must be a column-major float64 CUDA matrix")
+    if t.numel() == 0:  # empty views carry no meaningful strides
+        return max(t.shape[0], 1)
+    if t.stride(0) != 1:
         raise ValueError(f"{name} must be a column-major float64 CUDA matrix")
     return max(t.stride(1), t.shape[0], 1)
 
@@ -300,3 +308,21 @@ def bdsvd_vectors(alpha, beta):
     Y = np.zeros((n, n), order="F")
     _check(load().bdsvd_vectors_gpu(n, _p(alpha), _p(beta), _p(sigma), _p(X), _p(Y)))
     return sigma, X, Y
+
+
+def dgemm_torch(transa: str, transb: str, A, B, C, alpha: float = 1.0, beta: float = 0.0,
+                stream=None) -> None:
+    """C = alpha op(A) op(B) + beta C through svd_gpu_dgemm (the library's
+    FP64 MFMA GEMM) on column-major float64 CUDA tensors; C is updated in
+    place."""
+    torch = _torch()
+    ta, tb = transa.upper() == "T", transb.upper() == "T"
+    M, N = C.shape
+    K = A.shape[0] if ta else A.shape[1]
+    lda, ldb, ldc = _colmajor_ld(A, "A"), _colmajor_ld(B, "B"), _colmajor_ld(C, "C")
+    if stream is None:
+        stream = torch.cuda.current_stream(C.device).cuda_stream
+    _check(load().svd_gpu_dgemm(b"T" if ta else b"N", b"T" if tb else b"N", M, N, K, alpha,
+                                ctypes.c_void_p(A.data_ptr()), lda,
+                                ctypes.c_void_p(B.data_ptr()), ldb, beta,
+                                ctypes.c_void_p(C.data_ptr()), ldc, ctypes.c_void_p(stream)))

@@ -222,6 +222,18 @@ int svd_gpu_batched_noalloc(int count, const int *m, const int *n, int dtype,
                             const void *dA, void *dsigma, void *dU, void *dV,
                             int *dinfo, void *stream);
 
+/*
+ * The library's FP64 GEMM on the matrix cores (no reference counterpart; the
+ * kernel of the blocked trailing update and of the back-transformation),
+ * exposed for callers and tests: C = alpha op(A) op(B) + beta C on device
+ * column-major matrices, op(X) = X for 'N', X^T for 'T'; op(A) is M x K,
+ * op(B) K x N.  Ordered on stream (NULL = the null stream); returns once
+ * enqueued.
+ */
+int svd_gpu_dgemm(char transa, char transb, int M, int N, int K, double alpha,
+                  const double *dA, int lda, const double *dB, int ldb, double beta,
+                  double *dC, int ldc, void *stream);
+
 /* Milliseconds of the stages of the last successful svd_gpu / svd_gpu_status
  * / svd_gpu_dev call in this process: [0] copy-in, [1] bidiagonalization,
  * [2] singular values, [3] vectors, [4] back-transformation, [5] copy-out.
