@@ -694,6 +694,52 @@ __global__ __launch_bounds__(256) void bd_final_left(StepArgs<TS> a) {
   if (blockIdx.x == 0 && tid == 0) a.alpha[k] = R.diag();
 }
 
+// Wide matrices (m < n), the last row k = m-1 (bidiag.c:133-160,
+// bidiag_par.c:361-383): no left reflection -- alpha[k] is M[k, k], which is
+// then set to zero -- and the right reflector of row k over columns
+// k+1..n-1, giving the N x (N+1) upper bidiagonal.  Phase 1: row k of M on
+// those columns (A0[k, j] minus the panel terms of the earlier steps of the
+// panel) and a partial |r|^2 per work-group.
+template <typename TS>
+__global__ __launch_bounds__(FIN) void bd_wide_row(StepArgs<TS> a) {
+  __shared__ double uk[NB], xk[NB];
+  const int tid = threadIdx.x;
+  const int k = a.k, t = a.t;
+  const int j = k + 1 + blockIdx.x * FIN + tid;
+  const bool jv = j < a.n;
+  const int jc = jv ? j : k + 1;  // k + 1 <= n - 1: a valid column
+  if (tid < t) {
+    uk[tid] = (double)a.A[k + (int64_t)(a.p + tid) * a.lda];
+    xk[tid] = a.X[k + (int64_t)tid * a.lda];
+  }
+  __syncthreads();
+  double r = (double)a.A[k + (int64_t)jc * a.lda];
+  for (int s = 0; s < t; ++s)
+    r -= uk[s] * a.Y[jc + (int64_t)s * a.ldy] + xk[s] * a.V[jc + (int64_t)s * a.ldy];
+  if (jv) a.rvec[j] = r;
+  const double s2 = wave_sum(jv ? r * r : 0.0);
+  if (tid == 0) a.rnorm[blockIdx.x] = s2;
+}
+
+// Phase 2: the reflector into A[k, k+1:n], beta[k], alpha[k] = M[k, k]
+// (cvec[k], the column the previous step's right_fin left) and A[k, k] = 0.
+template <typename TS>
+__global__ __launch_bounds__(FIN) void bd_wide_fin(StepArgs<TS> a, int nrn) {
+  const int tid = threadIdx.x;
+  const int k = a.k;
+  double rn = 0.0;
+  for (int i = tid; i < nrn; i += FIN) rn += a.rnorm[i];
+  Refl R;
+  R.make(wave_sum(rn), a.rvec[k + 1]);
+  const int j = k + 1 + blockIdx.x * FIN + tid;
+  if (j < a.n) a.A[k + (int64_t)j * a.lda] = (TS)R.vec(a.rvec[j], j == k + 1);
+  if (blockIdx.x == 0 && tid == 0) {
+    a.beta[k] = R.diag();
+    a.alpha[k] = a.cvec[k];
+    a.A[k + (int64_t)k * a.lda] = (TS)0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 
 struct BidiagBufs {
@@ -705,7 +751,7 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
   d.ldy = (int)round_up(n, 16);
   d.nchL_max = (int)ceil_div(lda, RC) + 1;
   d.nccR_max = (int)ceil_div(n, CCR) + 1;
-  d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(m, FIN)) + 1;
+  d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(std::max(m, n), FIN)) + 1;
   d.nrn_max = (int)ceil_div(n, FIN) + 1;
   return d;
 }
@@ -794,7 +840,7 @@ void bidiag_plan_f32(ArenaPlan& p, int m, int n, int lda, int upd) {
 template <typename TS>
 static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* beta, int upd,
                         Arena& ws, hipStream_t st) {
-  if (m < n || n < 1 || lda < m || (lda % 2) != 0) return DDC_E_ARG;
+  if (m < 1 || n < 1 || lda < m || (lda % 2) != 0) return DDC_E_ARG;
   BidiagBufs d = bidiag_dims(m, n, lda);
   StepArgs<TS> a;
   a.m = m;
@@ -845,10 +891,19 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
   hipLaunchKernelGGL(bd_init<TS>, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
   DDC_LAUNCHED();
 
-  for (int k = 0; k < n; ++k) {
+  const int mn = std::min(m, n);
+  for (int k = 0; k < mn; ++k) {
     a.k = k;
     a.p = (k / NB) * NB;
     a.t = k - a.p;
+    if (m < n && k == m - 1) {
+      const int nw = (int)ceil_div(n - k - 1, FIN);
+      hipLaunchKernelGGL(bd_wide_row<TS>, dim3(nw), dim3(FIN), 0, st, a);
+      DDC_LAUNCHED();
+      hipLaunchKernelGGL(bd_wide_fin<TS>, dim3(nw), dim3(FIN), 0, st, a, nw);
+      DDC_LAUNCHED();
+      break;
+    }
     if (k == n - 1) {
       hipLaunchKernelGGL(bd_final_left<TS>, dim3((unsigned)ceil_div(m - k, 256)), dim3(256), 0,
                          st, a);
@@ -870,7 +925,9 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     DDC_LAUNCHED();
     // row k of M (the left finalization), the right reflector of row k and
     // the right GEMV
-    const int nfr = (int)ceil_div(m - k - 1, FIN);
+    // right_fin's grid covers the rows k+1..m-1 and, for the right
+    // reflector, the columns k+1..n-1
+    const int nfr = (int)ceil_div(std::max(m, n) - k - 1, FIN);
     if (k == n - 2) {
       const int nfl = (int)ceil_div(n - k - 1, FIN);
       hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);

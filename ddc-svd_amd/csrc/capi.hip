@@ -301,14 +301,23 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
   hipStream_t st = g_ctx.st;
   const bool wide = m < n;
   const int mt = wide ? n : m, nt = wide ? m : n;  // tall problem dims
+  // a wide A is factored through its transpose; its own bidiagonalization,
+  // in the reference's m x (m+1) layout, is run afterwards on a padded copy
+  // (Aw, leading dimension ldw) so that A is left holding the reflectors
+  const int ldw = (int)round_up(m, 16);
   ArenaPlan plan;
   tall_plan(plan, mt, nt, true);
-  if (wide) plan.add<double>((size_t)m * n);
+  if (wide) {
+    plan.add<double>((size_t)ldw * n);
+    plan.add<double>(2 * (size_t)n);
+    bidiag_plan(plan, m, n, ldw);
+  }
   Arena ws;
   DDC_TRY(arena_get(plan.bytes, ws, st));
   TallBufs b;
   DDC_TRY(tall_take(ws, mt, nt, true, b));
-  double* Araw = wide ? ws.take<double>((size_t)m * n) : nullptr;
+  double* Aw = wide ? ws.take<double>((size_t)ldw * n) : nullptr;
+  double* wab = wide ? ws.take<double>(2 * (size_t)n) : nullptr;
   Timer tm;
   DDC_TRY(tm.create(st));
   KClockScope kcs;
@@ -318,10 +327,13 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
     DDC_HIP(hipMemcpy2DAsync(b.Ap, sizeof(double) * b.lda_p, A, sizeof(double) * m,
                              sizeof(double) * m, n, hipMemcpyHostToDevice, st));
   } else {
-    DDC_HIP(hipMemcpyAsync(Araw, A, sizeof(double) * (size_t)m * n, hipMemcpyHostToDevice, st));
-    DDC_TRY(transpose_f64(m, n, Araw, m, b.Ap, b.lda_p, st));
+    DDC_HIP(hipMemsetAsync(Aw, 0, sizeof(double) * (size_t)ldw * n, st));
+    DDC_HIP(hipMemcpy2DAsync(Aw, sizeof(double) * ldw, A, sizeof(double) * m,
+                             sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+    DDC_TRY(transpose_f64(m, n, Aw, ldw, b.Ap, b.lda_p, st));
   }
   DDC_TRY(tall_stages(mt, nt, b, ws, b.U, mt, b.V, nt, st, tm));
+  if (wide) DDC_TRY(bidiag_run(m, n, Aw, ldw, wab, wab + n, ws, st));
   // outputs
   DDC_HIP(hipMemcpyAsync(sigma, b.sigma, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
   if (!wide) {
@@ -334,6 +346,8 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
     // A^T = U' S V'^T  =>  A = V' S U'^T : U = V' (m x m), V = U' (n x m)
     DDC_HIP(hipMemcpyAsync(U, b.V, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost, st));
     DDC_HIP(hipMemcpyAsync(V, b.U, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, Aw, sizeof(double) * ldw,
+                             sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
   }
   DDC_TRY(tm.mark());
   DDC_HIP(hipStreamSynchronize(st));
@@ -412,11 +426,10 @@ static int svd_dev(int m, int n, const double* dA, int lda, double* dsigma, doub
 
 static int bidiag_host(int m, int n, double* A, double* alpha, double* beta) {
   DDC_TRY(check_dims(m, n));
-  if (m < n) {
-    set_err("bidiag_gpu requires m >= n");
-    return DDC_E_ARG;
-  }
-  if (!A || !alpha || (n > 1 && !beta)) {
+  // beta has n-1 entries for m >= n and m entries for m < n (the N x (N+1)
+  // upper bidiagonal of a wide matrix, bidiag.c:133-160)
+  const int nbeta = m >= n ? n - 1 : m;
+  if (!A || !alpha || (nbeta > 0 && !beta)) {
     set_err("null pointer argument");
     return DDC_E_ARG;
   }
@@ -439,9 +452,9 @@ static int bidiag_host(int m, int n, double* A, double* alpha, double* beta) {
   DDC_TRY(bidiag_run(m, n, Ap, lda_p, da, db, ws, st));
   DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, Ap, sizeof(double) * lda_p,
                            sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
-  DDC_HIP(hipMemcpyAsync(alpha, da, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  if (n > 1)
-    DDC_HIP(hipMemcpyAsync(beta, db, sizeof(double) * (n - 1), hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipMemcpyAsync(alpha, da, sizeof(double) * std::min(m, n), hipMemcpyDeviceToHost, st));
+  if (nbeta > 0)
+    DDC_HIP(hipMemcpyAsync(beta, db, sizeof(double) * nbeta, hipMemcpyDeviceToHost, st));
   DDC_HIP(hipStreamSynchronize(st));
   return DDC_OK;
 }

@@ -265,10 +265,13 @@ def bidiag(A: np.ndarray):
     """bidiag_gpu: returns (A_with_reflectors, alpha, beta)."""
     A = np.array(A, dtype=np.float64, order="F", copy=True)
     m, n = A.shape
-    alpha = np.zeros(n)
-    beta = np.zeros(max(n - 1, 1))
+    # m >= n: n x n upper bidiagonal (beta: n-1 entries); m < n: the
+    # m x (m+1) upper bidiagonal of the reference (beta: m entries)
+    nbeta = n - 1 if m >= n else m
+    alpha = np.zeros(min(m, n))
+    beta = np.zeros(max(nbeta, 1))
     _check(load().bidiag_gpu(m, n, _p(A), _p(alpha), _p(beta)))
-    return A, alpha, beta[: n - 1]
+    return A, alpha, beta[:nbeta]
 
 
 def form_u(A_mod: np.ndarray) -> np.ndarray:

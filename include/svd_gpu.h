@@ -64,7 +64,11 @@ int svd_gpu_dev(int m, int n, const double *dA, int lda, double *dsigma,
  * (bidiag_par.h:30, bidiag_par.c:34-450).  Host buffers.  A is overwritten
  * with the unit-norm reflectors exactly as the reference stores them
  * (columns below/on the diagonal: left, rows right of the diagonal: right);
- * alpha gets min(m,n) entries, beta min(m,n)-1 entries.  Requires m >= n.
+ * alpha gets min(m,n) entries.  m >= n: beta gets n-1 entries (n x n upper
+ * bidiagonal).  m < n: beta gets m entries, the m x (m+1) upper bidiagonal of
+ * the reference (bidiag.c:133-160, bidiag_par.c:361-383): the last row takes
+ * a right reflector over columns m..n-1 and no left one, alpha[m-1] is taken
+ * as it stands and A[m-1, m-1] is set to zero.
  */
 int bidiag_gpu(int m, int n, double *A, double *alpha, double *beta);
 

@@ -87,6 +87,44 @@ def test_bidiag_matches_serial_reference(ddcsvd, po, shape):
     assert np.max(np.abs(Ag - Ao)) <= 1e-11
 
 
+WIDE_SHAPES = [(1, 2), (1, 5), (2, 3), (3, 5), (5, 6), (33, 100), (64, 96), (200, 700),
+               (130, 300)]
+
+
+@pytest.mark.parametrize("shape", WIDE_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_bidiag_wide_matches_serial_reference(ddcsvd, po, shape):
+    """m < n: the m x (m+1) upper bidiagonal and the reflector layout of
+    bidiag.c:133-160 (the oracle handles n >= m + 1 as the reference does)."""
+    m, n = shape
+    A = gen.uniform_matrix(m, n, 3 + m + n)
+    Ao, ao, bo = po.oracle_bidiag(A)
+    gpu_case(f"bidiag_gpu wide {m}x{n}")
+    Ag, ag, bg = ddcsvd.bidiag(A)
+    assert ag.shape == (m,) and bg.shape == (m,)
+    scale = np.max(np.abs(A))
+    assert np.max(np.abs(ag - ao)) <= 1e-12 * scale * max(1, n)
+    assert np.max(np.abs(bg - bo)) <= 1e-12 * scale * max(1, n)
+    assert np.max(np.abs(Ag - Ao)) <= 1e-11
+    # the m x (m+1) bidiagonal has the singular values of A
+    B = np.zeros((m, m + 1))
+    B[np.arange(m), np.arange(m)] = ag
+    B[np.arange(m), np.arange(1, m + 1)] = bg
+    sb = np.linalg.svd(B, compute_uv=False)
+    sa = np.linalg.svd(A, compute_uv=False)
+    assert np.max(np.abs(sb - sa)) <= 1e-13 * sa[0] * max(1, n)
+
+
+def test_svd_gpu_wide_leaves_reflectors_in_A(ddcsvd, po):
+    """The drop-in call leaves a wide A holding its reflectors in the
+    reference's m x (m+1) layout (bidiag_par.c:361-383, 405-418)."""
+    A = gen.uniform_matrix(96, 160, 21)
+    gpu_case("svd_gpu 96x160 A after the call")
+    s, U, V, Aout = ddcsvd.svd(A)
+    Ao, _, _ = po.oracle_bidiag(A)
+    assert np.max(np.abs(Aout - Ao)) <= 1e-11
+    check_svd(A, s, U, V)
+
+
 def test_bidiag_matches_lapack_dgebrd(ddcsvd, po):
     m, n = 200, 150
     A = gen.normal_matrix(m, n, 5)
