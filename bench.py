@@ -476,12 +476,19 @@ def mixed_form(n: int = 8192, seed: int = 3) -> dict:
     (s, U, V), t = timed(lambda: ddcsvd.svd_torch(A))
     out["plain_fp64"] = report(s, U, V, t)
     progress("mixed: plain FP64 done")
-    for name, upd, steps in (("f32_1step", "f32", 1), ("f32_2steps", "f32", 2),
-                             ("bf16x3_1step", "bf16x3", 1)):
-        (s, U, V, dg), t = timed(lambda: ddcsvd.svd_mixed_torch(A, upd, steps))
+    for name, upd, steps, rsv in (("f32_1step", "f32", 1, False),
+                                  ("f32_2steps", "f32", 2, False),
+                                  ("bf16x3_1step", "bf16x3", 1, False),
+                                  ("bf16_1step", "bf16", 1, False),
+                                  ("f32_1step_resolve", "f32", 1, True),
+                                  ("f32_2steps_resolve", "f32", 2, True),
+                                  ("bf16x3_1step_resolve", "bf16x3", 1, True),
+                                  ("bf16_1step_resolve", "bf16", 1, True)):
+        (s, U, V, dg), t = timed(lambda: ddcsvd.svd_mixed_torch(A, upd, steps, resolve=rsv))
         out[name] = report(s, U, V, t)
         out[name]["stages"] = dg
-        progress(f"mixed: {name} done ({1e3 * t:.0f} ms)")
+        out[name]["speedup_vs_plain"] = out["plain_fp64"]["ms"] / out[name]["ms"]
+        progress(f"mixed: {name} done ({1e3 * t:.0f} ms, residual {out[name]['residual']:.1e})")
     out["speedup_f32_1step_vs_plain"] = out["plain_fp64"]["ms"] / out["f32_1step"]["ms"]
     del A, s, U, V
     out["gaussian"] = mixed_gaussian(n)

@@ -278,6 +278,26 @@ static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu
   return DDC_OK;
 }
 
+// Workspace and body of an FP64 SVD of a device matrix (m >= n) inside
+// another stage, on the caller's arena and stream (used by the mixed form
+// for the block of singular values its refinement did not resolve).
+void svd_device_plan(ArenaPlan& p, int m, int n) { tall_plan(p, m, n, false); }
+
+int svd_device(int m, int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
+               double* dV, int ldv, Arena& ws, hipStream_t st) {
+  if (m < n || n < 1 || lda < m || ldu < m || ldv < n) return DDC_E_ARG;
+  TallBufs b;
+  DDC_TRY(tall_take(ws, m, n, false, b));
+  Timer tm;
+  DDC_TRY(tm.create(st));
+  DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * n, st));
+  DDC_HIP(hipMemcpy2DAsync(b.Ap, sizeof(double) * b.lda_p, dA, sizeof(double) * lda,
+                           sizeof(double) * m, n, hipMemcpyDeviceToDevice, st));
+  DDC_TRY(tall_stages(m, n, b, ws, dU, ldu, dV, ldv, st, tm));
+  DDC_HIP(hipMemcpyAsync(dsigma, b.sigma, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  return DDC_OK;
+}
+
 static int check_dims(int m, int n) {
   if (m < 1 || n < 1) {
     set_err("invalid dimensions m=%d n=%d", m, n);
@@ -582,7 +602,8 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
   const int flags = update & ~0xff;
   if (!dA || !dsigma || !dU || !dV || lda < n || ldu < n || ldv < n || steps < 0 || steps > 8 ||
       (upd != UPD_F32 && upd != UPD_BF16X3 && upd != UPD_BF16) ||
-      (flags & ~SVD_GPU_MIXED_SEPARATED) != 0) {
+      (flags & ~(SVD_GPU_MIXED_SEPARATED | SVD_GPU_MIXED_RESOLVE)) != 0 ||
+      ((flags & SVD_GPU_MIXED_RESOLVE) != 0 && steps < 1)) {
     set_err("invalid svd_gpu_mixed arguments");
     return DDC_E_ARG;
   }
@@ -590,7 +611,8 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
   hipStream_t st = user;  // NULL: the null stream, ordered with the caller's other work
   ArenaPlan plan;
   plan.add<int>(64);
-  mixed_plan(plan, n, upd);
+  const bool resolve = (flags & SVD_GPU_MIXED_RESOLVE) != 0;
+  mixed_plan(plan, n, upd, resolve);
   Arena ws;
   DDC_TRY(arena_get(plan.bytes, ws, st));
   int* flag = ws.take<int>(64);
@@ -606,7 +628,7 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
     return DDC_E_NUMERIC;
   }
   DDC_TRY(mixed_run(n, dA, lda, dsigma, dU, ldu, dV, ldv, upd,
-                    (flags & SVD_GPU_MIXED_SEPARATED) != 0, steps, diag, ws, st));
+                    (flags & SVD_GPU_MIXED_SEPARATED) != 0, resolve, steps, diag, ws, st));
   return DDC_OK;
 }
 

@@ -111,11 +111,19 @@ int bidiag_run_f32(int m, int n, float* A, int lda, double* alpha, double* beta,
 // Mixed-precision SVD of the n x n FP64 matrix A (device pointers): FP32
 // bidiagonalization with update mode upd, FP64 bidiagonal solve, FP32
 // back-transformation, `steps` FP64 refinement steps (separated: the larger
-// bound on the first-order corrections).  diag: see mixed.hip.
-void mixed_plan(ArenaPlan& p, int n, int upd);
+// bound on the first-order corrections; resolve: an FP64 Rayleigh-Ritz step
+// on the values whose pairs the last step left unresolved).  diag: see
+// mixed.hip.
+void mixed_plan(ArenaPlan& p, int n, int upd, bool resolve);
 int mixed_run(int n, const double* A, int lda, double* sigma, double* U, int ldu, double* V,
-              int ldv, int upd, bool separated, int steps, double* diag, Arena& ws,
-              hipStream_t st);
+              int ldv, int upd, bool separated, bool resolve, int steps, double* diag,
+              Arena& ws, hipStream_t st);
+
+// FP64 SVD of the m x n (m >= n) device matrix dA on the caller's arena and
+// stream (the plain pipeline; capi.hip): sigma ascending, U m x n, V n x n.
+void svd_device_plan(ArenaPlan& p, int m, int n);
+int svd_device(int m, int n, const double* dA, int lda, double* dsigma, double* dU, int ldu,
+               double* dV, int ldv, Arena& ws, hipStream_t st);
 
 // Singular values (ascending) of the n x (n+1) upper bidiagonal with diagonal
 // a[n] and super-diagonal b[n] by divide and conquer.

@@ -19,7 +19,12 @@
 //         F + F^T = R,  G + G^T = S,  T + F^T Sigma + Sigma G = Sigma
 //      gives F_ij, G_ij; then U <- U (I + F), V <- V (I + G).
 //      Each step is two FP64 GEMMs and two SYRKs (library, MFMA) for T, R
-//      and S, and two FP32 GEMMs for the update products U F and V G.
+//      and S, and two FP32 GEMMs for the update products U F and V G;
+//   4. optionally (resolve) an FP64 Rayleigh-Ritz step on the values whose
+//      pairs the last step could not separate: where a spectrum's absolute
+//      gaps fall below the FP32 stage's error (graded spectra) no number of
+//      first-order steps separates those vectors, the FP64 SVD of their
+//      projected block does.
 #include <algorithm>
 #include <vector>
 
@@ -129,7 +134,8 @@ __global__ void oa_sigma(int n, const double* T, const double* P, const double* 
 // reduced-precision stage and a first-order step would only mix them), get
 // the orthogonalising half alone.
 __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
-                      const double* sig, double guard2, double maxcorr, double* F, double* G) {
+                      const double* sig, double guard2, double maxcorr, double* F, double* G,
+                      int* rej) {
   const int64_t total = (int64_t)n * n;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -146,6 +152,7 @@ __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
     if (!(fabs(d) > guard2)) {
       F[e] = 0.5 * rij;
       G[e] = 0.5 * qij;
+      rej[i] = 1;
       continue;
     }
     const double tij = T[e], tji = T[j + (int64_t)i * n];
@@ -161,6 +168,31 @@ __global__ void oa_fg(int n, const double* T, const double* P, const double* Q,
                     fabs(g2) <= maxcorr;
     F[e] = ok ? f : 0.5 * rij;
     G[e] = ok ? g : 0.5 * qij;
+    // rej[i] = 1: index i belongs to a pair whose vectors the refinement
+    // could not separate (the pair marks i here and j from (j, i))
+    if (!ok) rej[i] = 1;
+  }
+}
+
+// D (n x k, ld n) = the columns idx[0..k-1] of S (ld lds).
+__global__ void mx_gather(int n, int k, const int* idx, const double* S, int lds, double* D) {
+  const int64_t total = (int64_t)n * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / n, i = e - c * n;
+    D[e] = S[i + (int64_t)idx[c] * lds];
+  }
+}
+
+// The columns idx[0..k-1] of D (ld ldd) = S (n x k, ld n); sd[idx[c]] = ss[c].
+__global__ void mx_scatter(int n, int k, const int* idx, const double* S, double* D, int ldd,
+                           const double* ss, double* sd) {
+  const int64_t total = (int64_t)n * k;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = e / n, i = e - c * n;
+    D[i + (int64_t)idx[c] * ldd] = S[e];
+    if (ss && i == 0) sd[idx[c]] = ss[c];
   }
 }
 
@@ -183,7 +215,7 @@ static int mixed_lda(int n) {
   return (l % 512 == 0) ? l + 16 : l;
 }
 
-void mixed_plan(ArenaPlan& p, int n, int upd) {
+void mixed_plan(ArenaPlan& p, int n, int upd, bool resolve) {
   const int lda_p = mixed_lda(n);
   const int ldb = (int)round_up(n, 16);
   p.add<float>((size_t)lda_p * n);
@@ -193,6 +225,8 @@ void mixed_plan(ArenaPlan& p, int n, int upd) {
   for (int i = 0; i < 8; ++i) p.add<double>((size_t)n * n);
   for (int i = 0; i < 3; ++i) p.add<float>((size_t)n * n * 2);
   p.add<double>(MX_PARTS);
+  p.add<int>(n);  // members of unresolved pairs
+  if (resolve) svd_device_plan(p, n, n);
   bidiag_plan_f32(p, n, n, lda_p, upd);
   dc_plan(p, n);
   twisted_plan(p, n);
@@ -215,14 +249,48 @@ static double host_defect(int n, const double* M, double* part, hipStream_t st, 
   return sqrt(s);
 }
 
+// Resolve: the members of the pairs the last refinement step could not separate
+// (their gaps lie below the FP32 stage's error; idx lists them, rej holds
+// room for them on the device) get a Rayleigh-Ritz step in FP64 on their own
+// left and right vectors,
+//   A_R = U_R^T A V_R = P S Q^T,  U_R <- U_R P,  V_R <- V_R Q,  sigma_R <- S.
+// The steps have already decoupled U_R, V_R from the other vectors, so the
+// FP64 SVD of the block A_R separates the members; the values
+// keep their ascending order because each block value lies within the
+// block's own error of the value it replaces.  Temporaries: UR, VR, X, URn
+// n x k; AR, Pb, Qb k x k; sb k.
+static int mixed_resolve(int n, const double* A, int lda, double* U, int ldu, double* V, int ldv,
+                         double* sigma, const std::vector<int>& idx, int* rej, double* UR,
+                         double* VR, double* X, double* AR, double* Pb, double* Qb, double* URn,
+                         double* sb, Arena& ws, hipStream_t st) {
+  const int k = (int)idx.size();
+  DDC_HIP(hipMemcpyAsync(rej, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, st));
+  const unsigned gk = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * k, 256), 8192);
+  hipLaunchKernelGGL(mx_gather, dim3(gk), dim3(256), 0, st, n, k, rej, U, ldu, UR);
+  hipLaunchKernelGGL(mx_gather, dim3(gk), dim3(256), 0, st, n, k, rej, V, ldv, VR);
+  DDC_LAUNCHED();
+  DDC_TRY(dgemm('N', 'N', n, k, n, 1.0, A, lda, VR, n, 0.0, X, n, st));
+  DDC_TRY(dgemm('T', 'N', k, k, n, 1.0, UR, n, X, n, 0.0, AR, k, st));
+  DDC_TRY(svd_device(k, k, AR, k, sb, Pb, k, Qb, k, ws, st));
+  DDC_TRY(dgemm('N', 'N', n, k, k, 1.0, UR, n, Pb, k, 0.0, URn, n, st));
+  DDC_TRY(dgemm('N', 'N', n, k, k, 1.0, VR, n, Qb, k, 0.0, X, n, st));
+  hipLaunchKernelGGL(mx_scatter, dim3(gk), dim3(256), 0, st, n, k, rej, URn, U, ldu, sb, sigma);
+  hipLaunchKernelGGL(mx_scatter, dim3(gk), dim3(256), 0, st, n, k, rej, X, V, ldv, nullptr,
+                     nullptr);
+  DDC_LAUNCHED();
+  return DDC_OK;
+}
+
 // diag (host, 8 entries, may be null): [0] ||I - U^T U||_F and [1]
 // ||I - V^T V||_F before the first refinement step, [2] bidiagonalization,
-// [3] values and vectors, [4] back-transformation, [5] refinement, [6] the
+// [3] values and vectors, [4] back-transformation, [5] refinement (resolve
+// included), [6] the
 // FP32 conversion, in milliseconds; [7] largest |change of sigma| / sigma_max
-// in the last refinement step.
+// in the last refinement step; with resolve (diag then holds 10 entries)
+// [8] the number of values resolved and [9] the resolve stage's milliseconds.
 int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int ldu, double* V,
-              int ldv, int upd, bool separated, int steps, double* diag, Arena& ws,
-              hipStream_t st) {
+              int ldv, int upd, bool separated, bool resolve, int steps, double* diag,
+              Arena& ws, hipStream_t st) {
   MixedBufs b;
   b.lda_p = mixed_lda(n);
   b.ldb = (int)round_up(n, 16);
@@ -239,14 +307,15 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
   b.F32 = ws.take<float>((size_t)n * n * 2);
   b.UF32 = ws.take<float>((size_t)n * n * 2);
   b.part = ws.take<double>(MX_PARTS);
-  if (!b.part) return DDC_E_NOMEM;
+  int* rej = ws.take<int>(n);
+  if (!rej) return DDC_E_NOMEM;
 
-  hipEvent_t ev[7];
+  hipEvent_t ev[8];
   for (auto& e : ev) DDC_HIP(hipEventCreate(&e));
   struct Ev {
     hipEvent_t* e;
     ~Ev() {
-      for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]);
+      for (int i = 0; i < 8; ++i) (void)hipEventDestroy(e[i]);
     }
   } evg{ev};
   DDC_HIP(hipEventRecord(ev[0], st));
@@ -275,6 +344,7 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
   DDC_HIP(hipEventRecord(ev[4], st));
 
   double defect_u = 0.0, defect_v = 0.0, dsig = 0.0;
+  std::vector<int> idx;  // the values resolved (host; read by the copy to rej)
   std::vector<double> s_old(n), s_new(n);
   double *Uc = b.U2, *Vc = b.V2, *X1 = b.W, *X2 = b.T;
   const dim3 gsy((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
@@ -307,8 +377,9 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
     dsig = smax > 0 ? dsig / smax : 0.0;
     const double g = 1e-13 * smax * smax;
     const unsigned gfg = (unsigned)std::min<int64_t>(ceil_div((int64_t)n * n, 256), 16384);
+    DDC_HIP(hipMemsetAsync(rej, 0, sizeof(int) * n, st));
     hipLaunchKernelGGL(oa_fg, dim3(gfg), dim3(256), 0, st, n, X2, b.P, b.Q, b.sigma, g,
-                       separated ? OA_MAXCORR_SEPARATED : OA_MAXCORR, b.F, b.G);
+                       separated ? OA_MAXCORR_SEPARATED : OA_MAXCORR, b.F, b.G, rej);
     DDC_LAUNCHED();
     // U <- U + U F, V <- V + V G into the free pair, or the outputs.  The
     // products are FP32 GEMMs: F and G are first-order corrections (entries
@@ -336,13 +407,32 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
       std::swap(Uc, X1);
       std::swap(Vc, X2);
     }
+    if (resolve && last) {
+      // after the last step: the steps have decoupled the members from the
+      // rest to second order by now (X1, X2 are free here).  Resolving after
+      // the first step instead, so that later steps refine the members with
+      // the rest, gave FP64-accurate values but a residual about 6x larger
+      // (8192^2 graded: 1.2e-10 against 2.0e-11 with two steps).
+      std::vector<int> hrej(n);
+      DDC_HIP(hipMemcpyAsync(hrej.data(), rej, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+      DDC_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i < n; ++i)
+        if (hrej[i]) idx.push_back(i);
+      DDC_HIP(hipEventRecord(ev[6], st));
+      if (!idx.empty())
+        DDC_TRY(mixed_resolve(n, A, lda, Un, ldun, Vn, ldvn, b.sigma, idx, rej, X1, X2, b.F, b.P,
+                              b.Q, b.G, reinterpret_cast<double*>(b.U32),
+                              reinterpret_cast<double*>(b.F32), ws, st));
+      DDC_HIP(hipEventRecord(ev[7], st));
+    }
   }
   DDC_HIP(hipMemcpyAsync(sigma_out, b.sigma, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   DDC_HIP(hipEventRecord(ev[5], st));
   DDC_HIP(hipStreamSynchronize(st));
   if (diag) {
-    float t[5] = {0, 0, 0, 0, 0};
+    float t[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+    if (resolve && steps > 0) (void)hipEventElapsedTime(&t[5], ev[6], ev[7]);
     diag[0] = defect_u;
     diag[1] = defect_v;
     diag[2] = t[1];
@@ -351,6 +441,10 @@ int mixed_run(int n, const double* A, int lda, double* sigma_out, double* U, int
     diag[5] = t[4];
     diag[6] = t[0];
     diag[7] = dsig;
+    if (resolve) {
+      diag[8] = (double)idx.size();
+      diag[9] = t[5];
+    }
   }
   return DDC_OK;
 }

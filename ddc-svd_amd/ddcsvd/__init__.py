@@ -232,16 +232,21 @@ def svd_backward_torch(U, sigma, V, gU, gsigma, gV, gap_guard: float = 1e-13):
 
 MIXED_UPDATES = {"f32": 1, "bf16x3": 2, "bf16": 3}
 MIXED_DIAG_KEYS = ["orth_u_before", "orth_v_before", "bidiag_ms", "values_vectors_ms",
-                   "backtransform_ms", "refine_ms", "convert_ms", "last_sigma_change"]
+                   "backtransform_ms", "refine_ms", "convert_ms", "last_sigma_change",
+                   "resolved", "resolve_ms"]
 
 
 MIXED_SEPARATED = 0x100
+MIXED_RESOLVE = 0x200
 
 
-def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1, separated: bool = False):
+def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1, separated: bool = False,
+                    resolve: bool = False):
     """svd_gpu_mixed on a square CUDA float64 tensor: (sigma, U, V, diag).
     separated: SVD_GPU_MIXED_SEPARATED (larger first-order corrections, for
-    spectra without long runs of close values)."""
+    spectra without long runs of close values); resolve: SVD_GPU_MIXED_RESOLVE
+    (FP64 Rayleigh-Ritz on the values the last step left unresolved; diag
+    then also holds "resolved" and "resolve_ms")."""
     import torch
     A = col_major(A)
     n = A.shape[0]
@@ -250,15 +255,17 @@ def svd_mixed_torch(A, update: str = "f32", refine_steps: int = 1, separated: bo
     sigma = torch.empty(n, dtype=torch.float64, device=A.device)
     U = col_major(torch.empty((n, n), dtype=torch.float64, device=A.device))
     V = col_major(torch.empty((n, n), dtype=torch.float64, device=A.device))
-    diag = np.zeros(8)
+    diag = np.zeros(10)
     stream = torch.cuda.current_stream(A.device).cuda_stream
     _check(load().svd_gpu_mixed(n, ctypes.c_void_p(A.data_ptr()), A.stride(1),
                                 ctypes.c_void_p(sigma.data_ptr()), ctypes.c_void_p(U.data_ptr()),
                                 U.stride(1), ctypes.c_void_p(V.data_ptr()), V.stride(1),
-                                MIXED_UPDATES[update] | (MIXED_SEPARATED if separated else 0),
+                                MIXED_UPDATES[update] | (MIXED_SEPARATED if separated else 0)
+                                | (MIXED_RESOLVE if resolve else 0),
                                 refine_steps, _p(diag),
                                 ctypes.c_void_p(stream)))
-    return sigma, U, V, dict(zip(MIXED_DIAG_KEYS, diag.tolist()))
+    keys = MIXED_DIAG_KEYS if resolve else MIXED_DIAG_KEYS[:8]
+    return sigma, U, V, dict(zip(keys, diag.tolist()))
 
 
 def bidiag(A: np.ndarray):

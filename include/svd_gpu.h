@@ -136,18 +136,28 @@ int svd_gpu_backward(int m, int n, const double *U, int ldu, const double *sigma
  *                close values come in isolated pairs (e.g. Gaussian
  *                matrices) two steps then reach FP64 accuracy; spectra with
  *                long runs of values closer than the FP32 stage's error
- *                (e.g. geometrically graded ones) lose orthogonality with it
- *   diag         optional host array of 8: [0] ||I-U^T U||_F, [1] ||I-V^T V||_F
- *                before refinement, [2] bidiagonalization, [3] values and
- *                vectors, [4] back-transformation, [5] refinement, [6] FP32
+ *                (e.g. geometrically graded ones) lose orthogonality with it;
+ *                and/or with SVD_GPU_MIXED_RESOLVE (needs refine_steps >= 1):
+ *                after the last step, the values of the pairs it left
+ *                unresolved (gaps below the FP32 stage's error) get a
+ *                Rayleigh-Ritz step in FP64, A_R = U_R^T A V_R = P S Q^T,
+ *                U_R <- U_R P, V_R <- V_R Q, with an FP64 SVD of the block
+ *                A_R
+ *   diag         optional host array of 8 (10 with SVD_GPU_MIXED_RESOLVE):
+ *                [0] ||I-U^T U||_F, [1] ||I-V^T V||_F before refinement,
+ *                [2] bidiagonalization, [3] values and vectors,
+ *                [4] back-transformation, [5] refinement (resolve
+ *                included), [6] FP32
  *                conversion (ms), [7] max |sigma change| / sigma_max in the
- *                last refinement step
+ *                last refinement step, [8] the number of values resolved,
+ *                [9] the resolve stage (ms)
  * Synchronous: returns after the results are complete.
  */
 #define SVD_GPU_MIXED_F32 1
 #define SVD_GPU_MIXED_BF16X3 2
 #define SVD_GPU_MIXED_BF16 3
 #define SVD_GPU_MIXED_SEPARATED 0x100
+#define SVD_GPU_MIXED_RESOLVE 0x200
 int svd_gpu_mixed(int n, const double *dA, int lda, double *dsigma, double *dU,
                   int ldu, double *dV, int ldv, int update, int refine_steps,
                   double *diag, void *stream);

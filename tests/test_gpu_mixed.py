@@ -20,6 +20,10 @@ the graded case):
   Gaussian 1024^2, FP32 update, SVD_GPU_MIXED_SEPARATED, 2 steps:
                                  |dsigma|/sigma_max <= 1e-13, residual <= 1e-12,
                                  orthogonality <= 1e-11
+  graded 1024^2, FP32 update, 2 steps, SVD_GPU_MIXED_RESOLVE:
+                                 |dsigma|/sigma_max <= 1e-10, residual <= 1e-10
+                                 (measured 5.4e-12 and 3.2e-12), orthogonality
+                                 <= 1e-4 (measured 2.9e-6)
 """
 import ctypes
 
@@ -78,6 +82,47 @@ def test_mixed_graded_per_sigma_error(ddcsvd):
     assert res <= 1e-7
 
 
+def test_mixed_graded_resolve_reaches_residual_bar(ddcsvd):
+    # the graded spectrum's bottom values have absolute gaps below the FP32
+    # stage's error; SVD_GPU_MIXED_RESOLVE gives them an FP64 Rayleigh-Ritz
+    # step on their projected block after the refinement steps
+    n = 1024
+    gpu_case(f"mixed graded resolve {n}")
+    sig = 10.0 ** (-6.0 * np.arange(n) / (n - 1))
+    A = gen.graded_matrix(n, n, sig, seed=3)
+    exact = np.sort(sig)
+    At = torch.from_numpy(np.asfortranarray(A)).cuda()
+    s, U, V, diag = ddcsvd.svd_mixed_torch(At, "f32", 2, resolve=True)
+    torch.cuda.synchronize()
+    s, U, V = s.cpu().numpy(), U.cpu().numpy(), V.cpu().numpy()
+    err = np.abs(s - exact)
+    res = np.linalg.norm(A - (U * s) @ V.T) / np.linalg.norm(A)
+    orth = max(np.linalg.norm(U.T @ U - np.eye(n)), np.linalg.norm(V.T @ V - np.eye(n)))
+    print(f"[mixed] graded {n} resolve: {int(diag['resolved'])} values resolved in "
+          f"{diag['resolve_ms']:.1f} ms; max |dsigma|/sigma_max {err.max():.2e}, residual "
+          f"{res:.2e}, orth {orth:.2e}", flush=True)
+    assert 0 < diag["resolved"] < n
+    assert np.all(np.diff(s) >= 0)
+    assert err.max() <= 1e-10
+    assert res <= 1e-10
+    assert orth <= 1e-4
+
+
+def test_mixed_resolve_leaves_separated_spectra_alone(ddcsvd):
+    # a Gaussian spectrum has no unresolved pairs after two steps of the
+    # larger bound: the resolve stage finds nothing and changes nothing
+    n = 512
+    gpu_case(f"mixed resolve gaussian {n}")
+    A = gen.normal_matrix(n, n, seed=8)
+    At = torch.from_numpy(np.asfortranarray(A)).cuda()
+    s0, U0, V0, _ = ddcsvd.svd_mixed_torch(At, "f32", 2, separated=True)
+    s1, U1, V1, d1 = ddcsvd.svd_mixed_torch(At, "f32", 2, separated=True, resolve=True)
+    torch.cuda.synchronize()
+    print(f"[mixed] gaussian {n} resolve: {int(d1['resolved'])} values resolved", flush=True)
+    assert d1["resolved"] == 0
+    assert torch.equal(s0, s1) and torch.equal(U0, U1) and torch.equal(V0, V1)
+
+
 @pytest.mark.parametrize("update,steps,bound", [("bf16x3", 1, 1e-6), ("bf16", 0, 1e-2)])
 def test_mixed_bf16_updates(ddcsvd, update, steps, bound):
     n = 512
@@ -131,5 +176,8 @@ def test_mixed_rejects_bad_arguments(ddcsvd):
     assert lib.svd_gpu_mixed(0, p(A), 8, p(s), p(A), 8, p(A), 8, 1, 1, None, None) == ddcsvd.E_ARG
     assert lib.svd_gpu_mixed(8, p(A), 4, p(s), p(A), 8, p(A), 8, 1, 1, None, None) == ddcsvd.E_ARG
     assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1, 9, None, None) == ddcsvd.E_ARG
-    assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1 | 0x200, 1, None,
+    assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1 | 0x400, 1, None,
+                             None) == ddcsvd.E_ARG
+    # the resolve stage needs a refinement step to find the unresolved pairs
+    assert lib.svd_gpu_mixed(8, p(A), 8, p(s), p(A), 8, p(A), 8, 1 | 0x200, 0, None,
                              None) == ddcsvd.E_ARG
