@@ -672,16 +672,29 @@ def main() -> int:
             out["sigma_max"] = float(s[-1])
     del A, U, V, sigma
     torch.cuda.empty_cache()
-    # the drop-in host-buffer entry point (svd_gpu_status): PCIe copies included
-    t0 = time.perf_counter()
-    ddcsvd.svd(A_host)
-    host_ms = 1e3 * (time.perf_counter() - t0)
+    # the drop-in host-buffer entry point (svd_gpu_status): PCIe copies
+    # included; the caller's arrays exist (and are paged in) beforehand, as a
+    # C caller's do
+    hA = np.asfortranarray(A_host)
+    hs, hU, hV = np.ones(mn), np.ones((m, m), order="F"), np.ones((n, n), order="F")
+    host_runs = []
+    for _ in range(2):
+        hA2 = hA.copy(order="F")
+        t0 = time.perf_counter()
+        rc = ddcsvd.load().svd_gpu_status(m, n, ddcsvd._p(hA2), ddcsvd._p(hs), ddcsvd._p(hU),
+                                          ddcsvd._p(hV))
+        host_runs.append(1e3 * (time.perf_counter() - t0))
+        if rc != 0:
+            raise RuntimeError(f"svd_gpu_status returned {rc}")
+    host_ms = min(host_runs)
     ht = ddcsvd.last_timings()
     out["host_buffers"] = {"ms_per_svd": host_ms, "svd_per_s": 1e3 / host_ms,
+                           "runs_ms": host_runs,
                            "copy_in_ms": ht["copy_in"], "copy_out_ms": ht["copy_out"],
-                           "note": "svd_gpu_status with host A, sigma, U, V (one call; "
-                                   "includes the host-side U and V allocations of the "
-                                   "Python wrapper)"}
+                           "note": "svd_gpu_status on host A, sigma, U, V allocated and "
+                                   "touched beforehand (pageable memory); the library "
+                                   "stages the copies through pinned buffers; best of 2"}
+    del hA, hA2, hs, hU, hV
     progress(f"host-buffer entry point: {host_ms:.1f} ms")
     progress("checks done")
     if not args.no_forms:

@@ -386,8 +386,9 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   const bool jv = col < jn;
   const int jc = jv ? j0 + col : k + 1;  // a valid column for the loads of idle lanes
 
-  // A row block issues its first eight columns of A0 before the prologue
-  // (rows past the column clamped; unused), so that they stream during it.
+  // A row block (rows r0, r0 + 1 of each thread) issues its first eight
+  // columns of A0 before the prologue (rows past the column clamped; unused),
+  // so that they stream during it.
   const bool rowblk = (int)blockIdx.x < nrb;
   const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
   const TS* base = a.A + min(r0, a.lda - 2) + (int64_t)j0 * a.lda;
@@ -475,36 +476,64 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   __syncthreads();
 
   if (rowblk) {
-    if (r0 < a.lda) {
-      double h0 = 0.0, h1 = 0.0;
-      int q = 0;
-      if (jn >= 8) {
+    // Two batches of eight columns in flight: the next batch is issued
+    // before the current one is consumed (one batch at a time left each
+    // wave idle for a full memory latency per batch; 8192^2: 283 -> 278 ms
+    // per SVD in this kernel).  Sixteen loads issued at once and then
+    // consumed, or two row pairs per thread, were slower (296 and 301 ms).
+    double h0 = 0.0, h1 = 0.0;
+    double2 va[8], vb[8];
+    auto load8 = [&](double2 (&v)[8], int c0) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          h0 += v0[u].x * rs[u];
-          h1 += v0[u].y * rs[u];
+      for (int u = 0; u < 8; ++u) v[u] = ld2(base + (int64_t)(c0 + u) * a.lda);
+    };
+    auto use8 = [&](const double2 (&v)[8], int c0) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        h0 += v[u].x * rs[c0 + u];
+        h1 += v[u].y * rs[c0 + u];
+      }
+    };
+    int c = 0;
+    if (jn >= 8) {
+      // the prefetched first batch is consumed before the loop, so that the
+      // loop is entered with one batch (va) in flight, as it is repeated
+      const bool more = jn >= 16;
+      if (more) load8(va, 8);
+      __builtin_amdgcn_sched_barrier(0);
+      use8(v0, 0);
+      c = 8;
+      if (more) {
+        // at the top of the loop va holds (or is loading) columns c - 8 .. c - 1
+        c = 16;
+        while (true) {
+          if (c + 8 > jn) {
+            use8(va, c - 8);
+            break;
+          }
+          load8(vb, c);
+          __builtin_amdgcn_sched_barrier(0);
+          use8(va, c - 8);
+          c += 8;
+          if (c + 8 > jn) {
+            use8(vb, c - 8);
+            break;
+          }
+          load8(va, c);
+          __builtin_amdgcn_sched_barrier(0);
+          use8(vb, c - 8);
+          c += 8;
         }
-        q = 8;
       }
-      for (; q + 8 <= jn; q += 8) {
-        double2 v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = ld2(base + (int64_t)(q + u) * a.lda);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          h0 += v[u].x * rs[q + u];
-          h1 += v[u].y * rs[q + u];
-        }
-      }
-      for (; q < jn; ++q) {
-        const double2 v = ld2(base + (int64_t)q * a.lda);
-        h0 += v.x * rs[q];
-        h1 += v.y * rs[q];
-      }
-      double* out = a.Rpart + (int64_t)blockIdx.y * a.lda;
-      if (r0 >= k + 1 && r0 < a.m) out[r0] = h0;
-      if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
     }
+    for (; c < jn; ++c) {
+      const double2 v = ld2(base + (int64_t)c * a.lda);
+      h0 += v.x * rs[c];
+      h1 += v.y * rs[c];
+    }
+    double* out = a.Rpart + (int64_t)blockIdx.y * a.lda;
+    if (r0 >= k + 1 && r0 < a.m) out[r0] = h0;
+    if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
   } else {
     // owner: y_t and r, then the chunk's partial sums for right_fin in the
     // layout of Rdot: [s] y_s^T r (s <= t), [t + 1 + s] v_s^T r (s < t),
@@ -937,8 +966,8 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       DDC_LAUNCHED();
     } else {
       const int rbr = ((k + 1) / 64) * 64;
-      const int nrb = (int)ceil_div(m - rbr, RBR);
       const int ncc = (int)ceil_div(n - k - 1, CCR);
+      const int nrb = (int)ceil_div(m - rbr, RBR);
       if (kclock_on() && kclock_pair(KCLOCK_RIGHT_GEMV, &ke0, &ke1) == 0)
         hipExtLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st,
                               ke0, ke1, 0, a, nrb, rbr, nchL);

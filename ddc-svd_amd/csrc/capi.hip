@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -38,6 +39,11 @@ struct Ctx {
   hipStream_t st = nullptr;
   char* arena = nullptr;
   size_t cap = 0;
+  // pinned staging buffers of the host-buffer entry points, the events of
+  // the copies that last used them and whether those may still be running
+  char* stage[2] = {nullptr, nullptr};
+  hipEvent_t sev[2] = {nullptr, nullptr};
+  bool spend[2] = {false, false};
 } g_ctx;
 
 void set_err(const char* fmt, ...) {
@@ -158,6 +164,137 @@ static int arena_get(size_t bytes, Arena& ws, hipStream_t st) {
   // shows up in the results
   static const bool poison = getenv("DDCSVD_POISON_ARENA") != nullptr;
   if (poison) DDC_HIP(hipMemsetAsync(g_ctx.arena, 0xff, g_ctx.cap, st));
+  return DDC_OK;
+}
+
+// ---- host <-> device copies of the host-buffer entry points ----------------
+// A copy from or to pageable memory goes through two pinned staging buffers:
+// the CPU copies one slice between the caller's buffer and a staging buffer
+// while the DMA engine moves the other (a pageable hipMemcpy stages through
+// the runtime's own buffers at a fraction of the link's rate).
+
+constexpr size_t STAGE_BYTES = size_t(32) << 20;  // per staging buffer
+constexpr size_t STAGE_MIN = size_t(4) << 20;     // smaller copies go directly
+
+static int stage_init() {
+  if (g_ctx.stage[1]) return DDC_OK;
+  for (int b = 0; b < 2; ++b) {
+    if (!g_ctx.sev[b]) DDC_HIP(hipEventCreateWithFlags(&g_ctx.sev[b], hipEventDisableTiming));
+    if (!g_ctx.stage[b])
+      DDC_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_ctx.stage[b]), STAGE_BYTES,
+                            hipHostMallocDefault));
+  }
+  return DDC_OK;
+}
+
+static int stage_wait(int b) {
+  if (g_ctx.spend[b]) {
+    DDC_HIP(hipEventSynchronize(g_ctx.sev[b]));
+    g_ctx.spend[b] = false;
+  }
+  return DDC_OK;
+}
+
+// cols columns of width bytes: dst[c dpitch + i] = src[c spitch + i]; a
+// large slice is split over four host threads
+static void host_copy2d(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width,
+                        size_t cols) {
+  auto run = [=](size_t c0, size_t c1) {
+    if (dpitch == width && spitch == width) {
+      memcpy(dst + c0 * width, src + c0 * width, (c1 - c0) * width);
+    } else {
+      for (size_t c = c0; c < c1; ++c) memcpy(dst + c * dpitch, src + c * spitch, width);
+    }
+  };
+  const bool contiguous = dpitch == width && spitch == width;
+  const size_t bytes = width * cols;
+  if (bytes < (size_t(8) << 20)) {
+    run(0, cols);
+    return;
+  }
+  constexpr int NT = 4;
+  std::thread th[NT - 1];
+  if (contiguous) {
+    // split the bytes, whatever the column count
+    const size_t per = (bytes + NT - 1) / NT;
+    for (int t = 1; t < NT; ++t) {
+      const size_t b0 = std::min(bytes, t * per), b1 = std::min(bytes, (t + 1) * per);
+      th[t - 1] = std::thread([=] { memcpy(dst + b0, src + b0, b1 - b0); });
+    }
+    memcpy(dst, src, std::min(bytes, per));
+  } else {
+    const size_t per = (cols + NT - 1) / NT;
+    for (int t = 1; t < NT; ++t) {
+      const size_t c0 = std::min(cols, t * per), c1 = std::min(cols, (t + 1) * per);
+      th[t - 1] = std::thread(run, c0, c1);
+    }
+    run(0, std::min(cols, per));
+  }
+  for (auto& t : th)
+    if (t.joinable()) t.join();
+}
+
+// host (pageable) -> device, stream-ordered on st; the caller synchronizes
+static int copy_h2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                    size_t cols, hipStream_t st) {
+  if (width * cols < STAGE_MIN || width > STAGE_BYTES) {
+    DDC_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, cols, hipMemcpyHostToDevice, st));
+    return DDC_OK;
+  }
+  DDC_TRY(stage_init());
+  const size_t per = STAGE_BYTES / width;
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  for (size_t c0 = 0, i = 0; c0 < cols; ++i) {
+    const size_t nc = std::min(per, cols - c0);
+    const int b = (int)(i & 1);
+    DDC_TRY(stage_wait(b));
+    host_copy2d(g_ctx.stage[b], width, s + c0 * spitch, spitch, width, nc);
+    DDC_HIP(hipMemcpy2DAsync(d + c0 * dpitch, dpitch, g_ctx.stage[b], width, width, nc,
+                             hipMemcpyHostToDevice, st));
+    DDC_HIP(hipEventRecord(g_ctx.sev[b], st));
+    g_ctx.spend[b] = true;
+    c0 += nc;
+  }
+  return DDC_OK;
+}
+
+// device -> host (pageable); complete when it returns
+static int copy_d2h(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                    size_t cols, hipStream_t st) {
+  if (width * cols < STAGE_MIN || width > STAGE_BYTES) {
+    DDC_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, cols, hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipStreamSynchronize(st));
+    return DDC_OK;
+  }
+  DDC_TRY(stage_init());
+  const size_t per = STAGE_BYTES / width;
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  size_t pc0 = 0, pnc = 0;
+  int pb = -1;
+  for (size_t c0 = 0, i = 0; c0 < cols; ++i) {
+    const size_t nc = std::min(per, cols - c0);
+    const int b = (int)(i & 1);
+    DDC_TRY(stage_wait(b));
+    DDC_HIP(hipMemcpy2DAsync(g_ctx.stage[b], width, s + c0 * spitch, spitch, width, nc,
+                             hipMemcpyDeviceToHost, st));
+    DDC_HIP(hipEventRecord(g_ctx.sev[b], st));
+    g_ctx.spend[b] = true;
+    if (pb >= 0) {
+      // the previous slice, while this one is in flight
+      DDC_TRY(stage_wait(pb));
+      host_copy2d(d + pc0 * dpitch, dpitch, g_ctx.stage[pb], width, width, pnc);
+    }
+    pc0 = c0;
+    pnc = nc;
+    pb = b;
+    c0 += nc;
+  }
+  if (pb >= 0) {
+    DDC_TRY(stage_wait(pb));
+    host_copy2d(d + pc0 * dpitch, dpitch, g_ctx.stage[pb], width, width, pnc);
+  }
   return DDC_OK;
 }
 
@@ -343,31 +480,28 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
   KClockScope kcs;
   DDC_TRY(tm.mark());
   DDC_HIP(hipMemsetAsync(b.Ap, 0, sizeof(double) * (size_t)b.lda_p * nt, st));
+  const size_t D = sizeof(double);
   if (!wide) {
-    DDC_HIP(hipMemcpy2DAsync(b.Ap, sizeof(double) * b.lda_p, A, sizeof(double) * m,
-                             sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+    DDC_TRY(copy_h2d(b.Ap, D * b.lda_p, A, D * m, D * m, n, st));
   } else {
-    DDC_HIP(hipMemsetAsync(Aw, 0, sizeof(double) * (size_t)ldw * n, st));
-    DDC_HIP(hipMemcpy2DAsync(Aw, sizeof(double) * ldw, A, sizeof(double) * m,
-                             sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+    DDC_HIP(hipMemsetAsync(Aw, 0, D * (size_t)ldw * n, st));
+    DDC_TRY(copy_h2d(Aw, D * ldw, A, D * m, D * m, n, st));
     DDC_TRY(transpose_f64(m, n, Aw, ldw, b.Ap, b.lda_p, st));
   }
   DDC_TRY(tall_stages(mt, nt, b, ws, b.U, mt, b.V, nt, st, tm));
   if (wide) DDC_TRY(bidiag_run(m, n, Aw, ldw, wab, wab + n, ws, st));
   // outputs
-  DDC_HIP(hipMemcpyAsync(sigma, b.sigma, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
+  DDC_TRY(copy_d2h(sigma, D * nt, b.sigma, D * nt, D * nt, 1, st));
   if (!wide) {
-    DDC_HIP(hipMemcpyAsync(U, b.U, sizeof(double) * (size_t)m * n, hipMemcpyDeviceToHost, st));
-    DDC_HIP(hipMemcpyAsync(V, b.V, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, st));
+    DDC_TRY(copy_d2h(U, D * m, b.U, D * m, D * m, n, st));
+    DDC_TRY(copy_d2h(V, D * n, b.V, D * n, D * n, n, st));
     // A is left holding the reflectors, as bidiag_par leaves it (bidiag_par.c:415-418)
-    DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, b.Ap, sizeof(double) * b.lda_p,
-                             sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
+    DDC_TRY(copy_d2h(A, D * m, b.Ap, D * b.lda_p, D * m, n, st));
   } else {
     // A^T = U' S V'^T  =>  A = V' S U'^T : U = V' (m x m), V = U' (n x m)
-    DDC_HIP(hipMemcpyAsync(U, b.V, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost, st));
-    DDC_HIP(hipMemcpyAsync(V, b.U, sizeof(double) * (size_t)n * m, hipMemcpyDeviceToHost, st));
-    DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, Aw, sizeof(double) * ldw,
-                             sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
+    DDC_TRY(copy_d2h(U, D * m, b.V, D * m, D * m, m, st));
+    DDC_TRY(copy_d2h(V, D * n, b.U, D * n, D * n, m, st));
+    DDC_TRY(copy_d2h(A, D * m, Aw, D * ldw, D * m, n, st));
   }
   DDC_TRY(tm.mark());
   DDC_HIP(hipStreamSynchronize(st));

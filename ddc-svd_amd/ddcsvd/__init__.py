@@ -147,7 +147,9 @@ def svd(A: np.ndarray, raw: bool = False):
     if raw:
         return rc, sigma, U, V, A
     _check(rc)
-    return sigma, U[:, :mn].copy(order="F"), V[:, :mn].copy(order="F"), A
+    U = U if U.shape[1] == mn else U[:, :mn].copy(order="F")
+    V = V if V.shape[1] == mn else V[:, :mn].copy(order="F")
+    return sigma, U, V, A
 
 
 def svd_dev(A, sigma, U, V, stream=None) -> None:
