@@ -43,6 +43,7 @@ of the plain workload's own N(0,1) matrix (residual, orthogonality, speed).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -342,10 +343,15 @@ def backward_form(reps: int = 3) -> dict:
             "backward_ms": 1e3 * min(tb)}
 
 
-def pmc_pass(m: int, n: int, seed: int) -> int:
-    """Workload for one rocprofv3 --pmc pass: three 1 GiB device copies (the
-    calibration kernel of known traffic), then the n-1 left-GEMV launches of
-    one bidiagonalization alone (svd_gpu_profile_left_gemv)."""
+def pmc_pass(kernel: str, m: int, n: int, seed: int) -> int:
+    """Workload for one rocprofv3 --pmc pass.  kernel "left" / "right": three
+    1 GiB device copies (the calibration kernel of known traffic), then the
+    left-GEMV (n-1) or right-GEMV (n-2) launches of one bidiagonalization
+    alone (svd_gpu_profile_left_gemv / _right_gemv).  kernel "gemm": the
+    trailing-update GEMMs of one m x n bidiagonalization, C (m-r0 x n-r0,
+    ld of the SVD's working copy) -= [U X] [Y V]^T with inner dimension 64,
+    r0 = 32, 64, ..., through svd_gpu_dgemm (the hand-written MFMA kernel),
+    and one 8192^3 GEMM."""
     import numpy as np
     import torch
 
@@ -353,6 +359,34 @@ def pmc_pass(m: int, n: int, seed: int) -> int:
     from ddcsvd import gen
 
     ddcsvd.load()
+    if kernel == "gemm":
+        lda = (m + 15) // 16 * 16
+        lda = lda + 8 if lda % 256 == 0 else lda   # the SVD's working-copy layout
+        C = torch.randn(n * lda, dtype=torch.float64, device="cuda")
+        UX = torch.randn(lda * 64, dtype=torch.float64, device="cuda")
+        ldy = (n + 15) // 16 * 16
+        YV = torch.randn(ldy * 64, dtype=torch.float64, device="cuda")
+        lib = ddcsvd.load()
+        p = lambda t, off=0: ctypes.c_void_p(t.data_ptr() + 8 * off)  # noqa: E731
+        shapes = [(m - r0, n - r0) for r0 in range(32, n, 32)]
+        print(f"pmc pass: {len(shapes)} trailing-update GEMMs of {m}x{n}, K = 64", flush=True)
+        for rm, rn in shapes:
+            r0 = m - rm
+            rc = lib.svd_gpu_dgemm(b"N", b"T", rm, rn, 64, ctypes.c_double(-1.0),
+                                   p(UX, r0), lda, p(YV, r0), ldy, ctypes.c_double(1.0),
+                                   p(C, r0 + r0 * lda), lda, None)
+            if rc != 0:
+                raise RuntimeError(f"svd_gpu_dgemm returned {rc}")
+        torch.cuda.synchronize()
+        big = [torch.randn(8192 * 8192, dtype=torch.float64, device="cuda") for _ in range(3)]
+        rc = lib.svd_gpu_dgemm(b"N", b"N", 8192, 8192, 8192, ctypes.c_double(1.0),
+                               p(big[0]), 8192, p(big[1]), 8192, ctypes.c_double(0.0),
+                               p(big[2]), 8192, None)
+        if rc != 0:
+            raise RuntimeError(f"svd_gpu_dgemm returned {rc}")
+        torch.cuda.synchronize()
+        print("pmc pass done", flush=True)
+        return 0
     x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
     y = torch.empty_like(x)
     for _ in range(3):
@@ -360,20 +394,28 @@ def pmc_pass(m: int, n: int, seed: int) -> int:
     torch.cuda.synchronize()
     del x, y
     A = ddcsvd.col_major(torch.from_numpy(gen.normal_matrix(m, n, seed)).cuda())
-    print(f"pmc pass: {n - 1} left-GEMV launches, {m}x{n}", flush=True)
-    ddcsvd.profile_left_gemv(A, n - 1, torch.cuda.current_stream().cuda_stream)
+    stream = torch.cuda.current_stream().cuda_stream
+    if kernel == "left":
+        print(f"pmc pass: {n - 1} left-GEMV launches, {m}x{n}", flush=True)
+        ddcsvd.profile_left_gemv(A, n - 1, stream)
+    else:
+        print(f"pmc pass: {n - 2} right-GEMV launches, {m}x{n}", flush=True)
+        ddcsvd.profile_right_gemv(A, n - 2, stream)
     torch.cuda.synchronize()
     print("pmc pass done", flush=True)
     return 0
 
 
-def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
-    """Reads the counter CSVs of the two passes and writes
-    profiles/bd_left_gemv_{m}x{n}_traffic.json: HBM bytes per launch of
-    bd_left_gemv = (FETCH_SIZE + WRITE_SIZE) per dispatch, each scaled by
-    what the counter reads for the 1 GiB copy (known traffic 1 GiB each way)."""
+def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -> int:
+    """Reads the counter CSVs of the two passes over one GEMV kernel ("left"
+    or "right") and writes profiles/bd_{kernel}_gemv_{m}x{n}_traffic.json:
+    HBM bytes per launch = (FETCH_SIZE + WRITE_SIZE) per dispatch, each scaled
+    by what the counter reads for the 1 GiB copy (known traffic 1 GiB each
+    way)."""
     import csv
     import glob
+
+    kname = f"bd_{kernel}_gemv"
 
     def load(d, counter):
         files = glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True)
@@ -388,13 +430,16 @@ def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
                 names[did] = r["Kernel_Name"]
         return per, names
 
-    out = {"workload": f"the {n - 1} left-GEMV launches of one {m}x{n} N(0,1) bidiagonalization "
-                       "(svd_gpu_profile_left_gemv), seed 2",
+    launches = n - 1 if kernel == "left" else n - 2
+    alg = (left_gemv_bytes(m, n) if kernel == "left" else right_gemv_bytes(m, n)) / launches
+    out = {"workload": f"the {launches} {kernel}-GEMV launches of one {m}x{n} N(0,1) "
+                       f"bidiagonalization (svd_gpu_profile_{kernel}_gemv), seed 2",
            "commands": [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run --output-format csv "
-                        f"-- python3 bench.py --pmc-pass --m {m} --n {n}",
+                        f"-- python3 bench.py --pmc-pass {kernel} --m {m} --n {n}",
                         f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o run --output-format csv "
-                        f"-- python3 bench.py --pmc-pass --m {m} --n {n}",
-                        f"python3 bench.py --summarize-pmc {fetch_dir} {write_dir} --m {m} --n {n}"],
+                        f"-- python3 bench.py --pmc-pass {kernel} --m {m} --n {n}",
+                        f"python3 bench.py --summarize-pmc {kernel} {fetch_dir} {write_dir} "
+                        f"--m {m} --n {n}"],
            "calibration": "bytes per counter unit = 1 GiB / the counter's mean over the three "
                           "1 GiB device copies of the same pass (each reads 1 GiB and writes "
                           "1 GiB)"}
@@ -403,18 +448,18 @@ def summarize_pmc(fetch_dir: str, write_dir: str, m: int, n: int) -> int:
         copy = [v for k, v in per.items() if "opy" in names[k] or "elementwise" in names[k]]
         copy = sorted(copy)[-3:]   # the three 1 GiB copies are the largest copy dispatches
         cal = (1 << 30) / (sum(copy) / len(copy)) if copy else None   # bytes per counter unit
-        left = [v for k, v in per.items() if "bd_left_gemv" in names[k]]
+        vals = [v for k, v in per.items() if kname in names[k]]
         out[counter] = {"copy_counter_per_dispatch": sum(copy) / len(copy) if copy else None,
                         "bytes_per_unit": cal,
-                        "left_gemv_dispatches": len(left),
-                        "left_gemv_counter_mean": sum(left) / len(left) if left else None}
-        out[counter]["left_gemv_bytes_per_launch"] = (
-            out[counter]["left_gemv_counter_mean"] * cal if left and cal else None)
-    f_b = out["FETCH_SIZE"]["left_gemv_bytes_per_launch"]
-    w_b = out["WRITE_SIZE"]["left_gemv_bytes_per_launch"]
+                        f"{kernel}_gemv_dispatches": len(vals),
+                        f"{kernel}_gemv_counter_mean": sum(vals) / len(vals) if vals else None}
+        out[counter][f"{kernel}_gemv_bytes_per_launch"] = (
+            out[counter][f"{kernel}_gemv_counter_mean"] * cal if vals and cal else None)
+    f_b = out["FETCH_SIZE"][f"{kernel}_gemv_bytes_per_launch"]
+    w_b = out["WRITE_SIZE"][f"{kernel}_gemv_bytes_per_launch"]
     out["hbm_bytes_per_launch"] = (f_b or 0.0) + (w_b or 0.0) if f_b else None
-    out["algorithmic_bytes_per_launch"] = left_gemv_bytes(m, n) / (n - 1)
-    path = os.path.join(ROOT, "profiles", f"bd_left_gemv_{m}x{n}_traffic.json")
+    out["algorithmic_bytes_per_launch"] = alg
+    path = os.path.join(ROOT, "profiles", f"{kname}_{m}x{n}_traffic.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
@@ -541,10 +586,12 @@ def mixed_gaussian(n: int = 8192, seed: int = 2) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pmc-pass", action="store_true",
-                    help="workload of one rocprofv3 --pmc pass (copy calibration + one SVD)")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="write the traffic summary from two --pmc pass outputs")
+    ap.add_argument("--pmc-pass", choices=["left", "right", "gemm"],
+                    help="workload of one rocprofv3 --pmc pass (copy calibration + one "
+                         "GEMV kernel's launches, or the trailing-update GEMMs)")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("KERNEL", "FETCH_DIR", "WRITE_DIR"),
+                    help="write the traffic summary of a GEMV kernel (left or right) from two "
+                         "--pmc pass outputs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -557,9 +604,9 @@ def main() -> int:
     ap.add_argument("--no-forms", action="store_true")
     args = ap.parse_args()
     if args.summarize_pmc:
-        return summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.m, args.n)
+        return summarize_pmc(*args.summarize_pmc, args.m, args.n)
     if args.pmc_pass:
-        return pmc_pass(args.m, args.n, args.seed)
+        return pmc_pass(args.pmc_pass, args.m, args.n, args.seed)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(this build has no multi-GPU path)", file=sys.stderr)

@@ -366,9 +366,10 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
 // row blocks then stream A0[rows, chunk] r; the owner block stores y_t and r
 // and leaves the chunk's partial sums of |r|^2, y_s^T r (s <= t) and
 // v_s^T r (s < t) for right_fin.  This saves the launch of bd_left_fin.
-template <typename TS, int CC>
+template <typename TS, int CC, int PF>
 __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0, int nchL) {
   static_assert(CC == 128, "two threads per column of the chunk");
+  static_assert(PF % 8 == 0 && PF >= 8, "columns prefetched in batches of eight");
   constexpr int H = NB / 2;             // panel columns per half
   __shared__ double rs[CC];             // r on the chunk
   __shared__ double yv[CC];             // y_t on the chunk
@@ -380,22 +381,38 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   __shared__ double red[4][32];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k, t = a.t;
-  const int j0 = k + 1 + blockIdx.y * CC;
+  // (row block bx, column chunk by), XCD-aware: work-groups go to the eight
+  // XCDs round-robin by linear id, so the ids are permuted to give each XCD
+  // whole runs of consecutive (bx, by); the row blocks of a chunk then share
+  // one L2 for the chunk's panel values (Y, V, the left GEMV's partials)
+  // instead of each XCD fetching its own copy.
+  int bx, by;
+  {
+    const int gx = (int)gridDim.x, total = gx * (int)gridDim.y;
+    const int lin = (int)blockIdx.x + (int)blockIdx.y * gx;
+    const int q = total / 8;
+    const int L = lin < 8 * q ? (lin % 8) * q + lin / 8 : lin;
+    bx = L % gx;
+    by = L / gx;
+  }
+  const int j0 = k + 1 + by * CC;
   const int jn = min(CC, a.n - j0);
   const int col = tid & (CC - 1), half = tid >> 7;
   const bool jv = col < jn;
   const int jc = jv ? j0 + col : k + 1;  // a valid column for the loads of idle lanes
 
-  // A row block (rows r0, r0 + 1 of each thread) issues its first eight
+  // A row block (rows r0, r0 + 1 of each thread) issues its first PF
   // columns of A0 before the prologue (rows past the column clamped; unused),
   // so that they stream during it.
-  const bool rowblk = (int)blockIdx.x < nrb;
-  const int r0 = rb0 + blockIdx.x * RBR + 2 * tid;
+  const bool rowblk = bx < nrb;
+  const int r0 = rb0 + bx * RBR + 2 * tid;
   const TS* base = a.A + min(r0, a.lda - 2) + (int64_t)j0 * a.lda;
-  double2 v0[8];
+  double2 v0[PF / 8][8];
   if (rowblk) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v0[u] = ld2(base + (int64_t)min(u, jn - 1) * a.lda);
+    for (int i = 0; i < PF / 8; ++i)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v0[i][u] = ld2(base + (int64_t)min(8 * i + u, jn - 1) * a.lda);
   }
 
   // Every load first.  Thread (col, half) holds Y[j, s], V[j, s] for the
@@ -495,17 +512,18 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
       }
     };
     int c = 0;
-    if (jn >= 8) {
-      // the prefetched first batch is consumed before the loop, so that the
+    if (jn >= PF) {
+      // the prefetched batches are consumed before the loop, so that the
       // loop is entered with one batch (va) in flight, as it is repeated
-      const bool more = jn >= 16;
-      if (more) load8(va, 8);
+      const bool more = jn >= PF + 8;
+      if (more) load8(va, PF);
       __builtin_amdgcn_sched_barrier(0);
-      use8(v0, 0);
-      c = 8;
+#pragma unroll
+      for (int i = 0; i < PF / 8; ++i) use8(v0[i], 8 * i);
+      c = PF;
       if (more) {
         // at the top of the loop va holds (or is loading) columns c - 8 .. c - 1
-        c = 16;
+        c = PF + 8;
         while (true) {
           if (c + 8 > jn) {
             use8(va, c - 8);
@@ -531,7 +549,7 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
       h0 += v.x * rs[c];
       h1 += v.y * rs[c];
     }
-    double* out = a.Rpart + (int64_t)blockIdx.y * a.lda;
+    double* out = a.Rpart + (int64_t)by * a.lda;
     if (r0 >= k + 1 && r0 < a.m) out[r0] = h0;
     if (r0 + 1 >= k + 1 && r0 + 1 < a.m) out[r0 + 1] = h1;
   } else {
@@ -560,13 +578,13 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
       const double v = red[2 * hf][i] + red[2 * hf + 1][i];
       const bool ysl = i < H;
       const int sx = hf * H + (ysl ? i : i - H);
-      double* Rd = a.Rdot + (int64_t)blockIdx.y * DSTRIDE;
+      double* Rd = a.Rdot + (int64_t)by * DSTRIDE;
       if (ysl) {
         if (sx <= t) Rd[sx] = v;
       } else if (sx < t) {
         Rd[t + 1 + sx] = v;
       } else if (sx == NB - 1) {
-        a.rnorm[blockIdx.y] = v;
+        a.rnorm[by] = v;
       }
     }
   }
@@ -968,12 +986,14 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       const int rbr = ((k + 1) / 64) * 64;
       const int ncc = (int)ceil_div(n - k - 1, CCR);
       const int nrb = (int)ceil_div(m - rbr, RBR);
+      // 16 columns prefetched before the prologue (8: 1.5-2 ms more per
+      // 8192^2 SVD; 24 would lower the occupancy to two waves per SIMD)
+      auto kern = bd_right_gemv<TS, CCR, 16>;
       if (kclock_on() && kclock_pair(KCLOCK_RIGHT_GEMV, &ke0, &ke1) == 0)
-        hipExtLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st,
-                              ke0, ke1, 0, a, nrb, rbr, nchL);
+        hipExtLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, ke0, ke1, 0, a, nrb,
+                              rbr, nchL);
       else
-        hipLaunchKernelGGL((bd_right_gemv<TS, CCR>), dim3(nrb + 1, ncc), dim3(256), 0, st, a,
-                           nrb, rbr, nchL);
+        hipLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, a, nrb, rbr, nchL);
       DDC_LAUNCHED();
       a.nrn = ncc;
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
@@ -1067,6 +1087,70 @@ int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int s
     const int nchL = (int)ceil_div(m - rb0, RC);
     hipLaunchKernelGGL((bd_left_gemv<double, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0, st,
                        a, ncb, rb0);
+    DDC_LAUNCHED();
+  }
+  return DDC_OK;
+}
+
+// Measurement: `steps` right-GEMV launches, at column steps k = 0..steps-1,
+// after one left GEMV at k = 0 (m >= n; A's column 0 is overwritten with a
+// reflector).  Every launch reads the partial sums of that left GEMV and
+// the zeroed panel buffers, so no other kernel runs between them and a
+// counter pass sees the right GEMV's own traffic for the trailing blocks
+// A[k+1:m, k+1:n]; the values computed are discarded.
+int bidiag_right_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps,
+                           Arena& ws, hipStream_t st) {
+  if (m < n || n < 3 || lda < m || (lda % 2) != 0 || steps < 1 || steps > n - 2)
+    return DDC_E_ARG;
+  BidiagBufs d = bidiag_dims(m, n, lda);
+  StepArgs<double> a;
+  a.m = m;
+  a.n = n;
+  a.lda = lda;
+  a.ldy = d.ldy;
+  a.A = A;
+  a.alpha = alpha;
+  a.beta = nullptr;
+  a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
+  a.V = a.Y + (size_t)d.ldy * NB;
+  double* UX = ws.take<double>((size_t)lda * NB * 2);
+  a.X = UX ? UX + (size_t)lda * NB : nullptr;
+  a.Ucopy = UX;
+  a.cvec = ws.take<double>(lda);
+  a.rvec = ws.take<double>(d.ldy);
+  a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
+  a.Rpart = ws.take<double>((size_t)d.nccR_max * lda);
+  a.Ldot = ws.take<double>((size_t)d.nchL_max * 2 * NB);
+  a.Rdot = ws.take<double>((size_t)d.nccR_max * DSTRIDE);
+  a.cnorm = ws.take<double>(d.ncn_max);
+  a.rnorm = ws.take<double>(d.nrn_max);
+  if (!a.rnorm) return DDC_E_NOMEM;
+  DDC_HIP(hipMemsetAsync(a.Y, 0, sizeof(double) * (size_t)d.ldy * NB * 2, st));
+  DDC_HIP(hipMemsetAsync(UX, 0, sizeof(double) * (size_t)lda * NB * 2, st));
+  DDC_HIP(hipMemsetAsync(a.Ldot, 0, sizeof(double) * (size_t)d.nchL_max * 2 * NB, st));
+  a.ncn = (int)ceil_div(m, 256);
+  a.nrn = 0;
+  hipLaunchKernelGGL(bd_init<double>, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
+  DDC_LAUNCHED();
+  a.k = a.p = a.t = 0;
+  const int ncb0 = (int)ceil_div(n - 1, CBL);
+  const int nchL0 = (int)ceil_div(m, RC);
+  hipLaunchKernelGGL((bd_left_gemv<double, RC / 512>), dim3(ncb0, nchL0), dim3(256), 0, st, a,
+                     ncb0, 0);
+  DDC_LAUNCHED();
+  for (int k = 0; k < steps; ++k) {
+    a.k = k;
+    a.p = (k / NB) * NB;
+    a.t = k - a.p;
+    // the left GEMV's partials cover rows from 0: the chunk count of step k
+    // is at most that of step 0
+    const int rb0 = (k / 64) * 64;
+    const int nchL = (int)ceil_div(m - rb0, RC);
+    const int rbr = ((k + 1) / 64) * 64;
+    const int ncc = (int)ceil_div(n - k - 1, CCR);
+    const int nrb = (int)ceil_div(m - rbr, RBR);
+    hipLaunchKernelGGL((bd_right_gemv<double, CCR, 16>), dim3(nrb + 1, ncc), dim3(256), 0, st,
+                       a, nrb, rbr, nchL);
     DDC_LAUNCHED();
   }
   return DDC_OK;

@@ -510,11 +510,16 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
 }
 
 // Counter-collection workload: a padded copy of dA, then `steps` left-GEMV
-// launches of its bidiagonalization and nothing else.
-static int left_gemv_dev(int m, int n, const double* dA, int lda, int steps, hipStream_t user) {
+// (right: right-GEMV) launches of its bidiagonalization and nothing else.
+static int gemv_only_dev(bool right, int m, int n, const double* dA, int lda, int steps,
+                         hipStream_t user) {
   DDC_TRY(check_dims(m, n));
-  if (!dA || m < n || n < 2 || lda < m || steps < 1 || steps > n - 1) {
+  if (!right && (!dA || m < n || n < 2 || lda < m || steps < 1 || steps > n - 1)) {
     set_err("invalid arguments (need m >= n >= 2, lda >= m, 1 <= steps <= n-1)");
+    return DDC_E_ARG;
+  }
+  if (right && (!dA || m < n || n < 3 || lda < m || steps < 1 || steps > n - 2)) {
+    set_err("invalid arguments (need m >= n >= 3, lda >= m, 1 <= steps <= n-2)");
     return DDC_E_ARG;
   }
   DDC_TRY(ctx_init());
@@ -531,7 +536,10 @@ static int left_gemv_dev(int m, int n, const double* dA, int lda, int steps, hip
   DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
   DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, dA, sizeof(double) * lda,
                            sizeof(double) * m, n, hipMemcpyDeviceToDevice, st));
-  DDC_TRY(bidiag_left_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
+  if (right)
+    DDC_TRY(bidiag_right_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
+  else
+    DDC_TRY(bidiag_left_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
   DDC_HIP(hipStreamSynchronize(st));
   return DDC_OK;
 }
@@ -863,7 +871,14 @@ int svd_gpu_profile_left_gemv(int m, int n, const double* dA, int lda, int steps
                               void* stream) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_err[0] = 0;
-  return left_gemv_dev(m, n, dA, lda, steps, (hipStream_t)stream);
+  return gemv_only_dev(false, m, n, dA, lda, steps, (hipStream_t)stream);
+}
+
+int svd_gpu_profile_right_gemv(int m, int n, const double* dA, int lda, int steps,
+                               void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return gemv_only_dev(true, m, n, dA, lda, steps, (hipStream_t)stream);
 }
 
 int svd_gpu_profile_kernels(int on) {

@@ -95,6 +95,9 @@ void bidiag_plan(ArenaPlan& p, int m, int n, int lda);
 // Measurement: `steps` left-GEMV launches alone (A and alpha overwritten).
 int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps, Arena& ws,
                           hipStream_t st);
+// Measurement: `steps` right-GEMV launches (after one left GEMV at k = 0).
+int bidiag_right_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps,
+                           Arena& ws, hipStream_t st);
 int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta,
                Arena& ws, hipStream_t st);
 

@@ -83,6 +83,8 @@ def load():
         lib.svd_gpu_profile_kernels.restype = i
         lib.svd_gpu_profile_left_gemv.argtypes = [i, i, v, i, i, v]
         lib.svd_gpu_profile_left_gemv.restype = i
+        lib.svd_gpu_profile_right_gemv.argtypes = [i, i, v, i, i, v]
+        lib.svd_gpu_profile_right_gemv.restype = i
         lib.svd_gpu_last_error.argtypes = []
         lib.svd_gpu_last_error.restype = ctypes.c_char_p
         c = ctypes.c_char
@@ -130,6 +132,15 @@ def profile_left_gemv(A, steps: int, stream=None) -> None:
     lda = _colmajor_ld(A, "A")
     _check(load().svd_gpu_profile_left_gemv(m, n, ctypes.c_void_p(A.data_ptr()), lda, steps,
                                             ctypes.c_void_p(stream)))
+
+
+def profile_right_gemv(A, steps: int, stream=None) -> None:
+    """svd_gpu_profile_right_gemv on a column-major float64 CUDA matrix: the
+    first `steps` right-GEMV launches of its bidiagonalization, alone."""
+    m, n = A.shape
+    lda = _colmajor_ld(A, "A")
+    _check(load().svd_gpu_profile_right_gemv(m, n, ctypes.c_void_p(A.data_ptr()), lda, steps,
+                                             ctypes.c_void_p(stream)))
 
 
 def svd(A: np.ndarray, raw: bool = False):

@@ -270,6 +270,14 @@ int svd_gpu_profile_kernels(int on);
 int svd_gpu_profile_left_gemv(int m, int n, const double *dA, int lda, int steps,
                               void *stream);
 
+/* Measurement hook (no reference counterpart): the first `steps` right-GEMV
+ * launches (1 <= steps <= n-2) of the bidiagonalization of a copy of the
+ * m x n device matrix dA (m >= n >= 3), after one left GEMV of column 0 and
+ * with no other kernel in between: each launch streams the trailing block of
+ * its column step, as in the SVD.  Workspace only; results are discarded. */
+int svd_gpu_profile_right_gemv(int m, int n, const double *dA, int lda, int steps,
+                               void *stream);
+
 /* Human-readable description of the last failure (empty if none). */
 const char *svd_gpu_last_error(void);
 

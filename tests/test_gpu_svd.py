@@ -428,6 +428,26 @@ def test_profile_left_gemv_hook(ddcsvd):
                                                               device="cuda")), 1)
 
 
+def test_profile_right_gemv_hook(ddcsvd):
+    """The right-GEMV counter hook runs on a workspace copy (the caller's
+    matrix is unchanged) and rejects step counts outside 1..n-2 and shapes
+    with m < n or n < 3."""
+    import torch
+    A = gen.normal_matrix(700, 300, 36)
+    At = ddcsvd.col_major(torch.from_numpy(A).to("cuda"))
+    gpu_case("profile_right_gemv 700x300, 298 steps")
+    ddcsvd.profile_right_gemv(At, 298)
+    torch.cuda.synchronize()
+    assert np.array_equal(At.cpu().numpy(), A)
+    for steps in (0, 299):
+        with pytest.raises(ddcsvd.SvdGpuError):
+            ddcsvd.profile_right_gemv(At, steps)
+    for shape in ((3, 5), (4, 2)):
+        with pytest.raises(ddcsvd.SvdGpuError):
+            ddcsvd.profile_right_gemv(ddcsvd.col_major(torch.zeros(shape, dtype=torch.float64,
+                                                                   device="cuda")), 1)
+
+
 def test_svd_dev_null_stream_orders_after_torch_work(ddcsvd):
     """svd_gpu_dev with stream NULL (torch's default stream handle is 0) must
     run after the caller's pending work on that stream: A is produced by a
