@@ -466,6 +466,185 @@ def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -
     return 0
 
 
+def summarize_trace(trace_dir: str, bench_json: str, m: int, n: int) -> int:
+    """Reads the kernel trace of `rocprofv3 --kernel-trace --stats -- python3
+    bench.py --steps 3 --warmup 1 --no-cpu --no-forms` (nine m x n SVDs: one
+    warm-up, three timed, three kernel-timed, two through the host-buffer
+    entry point) and the bench line that run printed, and writes
+    profiles/svd_gpu_{m}x{n}_kernels.md: time per SVD by kernel, the
+    bidiagonalization kernels by column-step range, and the dominant kernel's
+    average launch against the bench line's roofline figure."""
+    import csv
+    import glob
+    import re
+
+    files = glob.glob(os.path.join(trace_dir, "**", "*kernel_trace*.csv"), recursive=True)
+    rows = []
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+
+    def short(name):
+        mm = re.search(r"ddc::(?:\(anonymous namespace\)::)?(\w+)(<[^()]*>)?", name)
+        if mm:
+            return "ddc::" + mm.group(1) + (mm.group(2) or "")
+        if name.startswith("Cijk") or "gemm" in name.lower():
+            mt = re.search(r"MT(\d+x\d+x\d+)", name)
+            return "rocBLAS gemm " + (mt.group(1) if mt else name[:40])
+        return name[:60]
+
+    steps = n - 1 if m >= n else m
+    nsvd = max(1, sum(1 for _, _, nm in rows if "bd_init" in nm))
+    agg = {}
+    by_range = {}
+    k = -1
+    for t0, t1, nm in rows:
+        sn = short(nm)
+        a = agg.setdefault(sn, [0, 0.0])
+        a[0] += 1
+        a[1] += (t1 - t0) / 1e3
+        if "bd_init" in nm:
+            k = -1
+        if "bd_left_gemv" in nm:
+            k += 1
+        for kern in ("bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
+            if kern in nm and k >= 0:
+                b = by_range.setdefault((kern, k // 1024), [0, 0.0])
+                b[0] += 1
+                b[1] += (t1 - t0) / 1e3
+    tot_us = sum(v[1] for v in agg.values())
+    out = [f"# Kernel times of the plain FP64 SVD, {m} x {n}, N(0,1) seed 2", "",
+           "Command (MI355X, ROCm 7.2, one GPU; run from the repository root with "
+           "`TMPDIR=/tmp`):", "",
+           f"    rocprofv3 --kernel-trace --stats -d {trace_dir} -o run --output-format csv -- "
+           f"python3 bench.py --steps 3 --warmup 1 --no-cpu --no-forms > {bench_json}",
+           f"    python3 bench.py --summarize-trace {trace_dir} {bench_json} --m {m} --n {n}", "",
+           f"All dispatches of the run ({len(rows)}; {nsvd} SVDs: one warm-up, three timed, "
+           "three kernel-timed, two through the host-buffer entry point, whose staged copies "
+           "are the copy rows), divided by the number of SVDs.", "",
+           "```",
+           f"{'kernel':52s} {'calls/SVD':>9s} {'ms/SVD':>8s} {'avg us':>9s} {'%':>5s}"]
+    for sn, (c, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
+        if us / nsvd < 50:   # under 0.05 ms per SVD
+            continue
+        out.append(f"{sn[:52]:52s} {c / nsvd:9.0f} {us / nsvd / 1e3:8.2f} {us / c:9.2f} "
+                   f"{100 * us / tot_us:5.1f}")
+    out.append(f"total kernel time per SVD: {tot_us / nsvd / 1e3:.1f} ms")
+    out += ["", "bidiagonalization kernels by column step (average us per launch)",
+            f"{'steps':>13s} {'bd_left_gemv':>14s} {'bd_right_gemv':>14s} {'bd_right_fin':>14s}"]
+    for rng in range((steps + 1023) // 1024):
+        cells = []
+        for kern in ("bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
+            c, us = by_range.get((kern, rng), (0, 0.0))
+            cells.append(f"{us / c:14.2f}" if c else f"{'-':>14s}")
+        lo, hi = 1024 * rng, min(1024 * rng + 1023, steps - 1)
+        out.append(f"{lo:6d}-{hi:6d} " + " ".join(cells))
+    out.append("```")
+    try:
+        line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+        rf = line["roofline"]
+        kern = rf["kernel"]
+        c, us = agg.get(next(sn for sn in agg if kern in sn), (0, 0.0))
+        out += ["", f"Agreement with the bench line of the same run: `roofline.avg_launch_us` = "
+                f"{rf['avg_launch_us']:.2f} us for `{kern}` (HIP event pairs around every launch "
+                f"in the kernel-timed steps); this trace: {us / c:.2f} us per launch over all "
+                f"{c} launches.  Algorithmic bytes per launch "
+                f"{rf['algorithmic_bytes_per_launch'] / 1e6:.1f} MB; counter traffic "
+                f"{(rf['traffic'] or 0) / 1e6:.1f} MB (profiles/{kern}_{m}x{n}_traffic.json); "
+                f"achieved {rf['achieved']:.0f} GB/s = {rf['frac']:.3f} of 8 TB/s."]
+    except (OSError, ValueError, KeyError, StopIteration):
+        pass
+    path = os.path.join(ROOT, "profiles", f"svd_gpu_{m}x{n}_kernels.md")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    print("\n".join(out))
+    return 0
+
+
+def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
+    """Reads the counter CSV of a --pmc-pass gemm run (MfmaUtil and
+    SQ_INSTS_VALU_MFMA_MOPS_F64 with the kernel trace) and writes
+    profiles/k_dgemm_{m}x{n}_mfma.md: per dispatch of the hand-written FP64
+    MFMA GEMM, MFMA FLOPs = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (the counter
+    definitions' MfmaFlopsF64), duration from the dispatch timestamps,
+    MfmaUtil = MFMA-busy cycles / (GPU-busy cycles x SIMDs)."""
+    import csv
+    import glob
+
+    files = glob.glob(os.path.join(trace_dir, "**", "*counter_collection*.csv"), recursive=True)
+    per = {}
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            d = per.setdefault((f, int(r["Dispatch_Id"])), {})
+            d[r["Counter_Name"]] = float(r["Counter_Value"])
+            d["name"] = r["Kernel_Name"]
+            d["ns"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    ks = [v for _, v in sorted(per.items()) if "k_dgemm" in v["name"]]
+    if len(ks) < 2:
+        print("no k_dgemm dispatches found", file=sys.stderr)
+        return 1
+    upd, big = ks[:-1], ks[-1]
+    peak = 78.6  # TFLOP/s, MI355X dense FP64 MFMA
+
+    def line(label, v, alg_flops, alg_bytes):
+        fl = 512.0 * v["SQ_INSTS_VALU_MFMA_MOPS_F64"]
+        us = v["ns"] / 1e3
+        return (f"| {label} | {us:.1f} | {alg_flops / 1e9:.2f} | {fl / 1e9:.2f} | "
+                f"{fl / us / 1e6:.1f} | {100 * fl / us / 1e6 / peak:.1f} | {v['MfmaUtil']:.1f} | "
+                f"{alg_bytes / us / 1e3:.0f} |")
+
+    shapes = [(m - r0, n - r0) for r0 in range(32, n, 32)]
+    out = [f"# FP64 MFMA GEMM (k_dgemm): trailing updates of a {m}x{n} bidiagonalization",
+           "",
+           "Command (MI355X, ROCm 7.2, one GPU; run from the repository root with "
+           "`TMPDIR=/tmp`):",
+           "",
+           f"    rocprofv3 --kernel-trace --pmc MfmaUtil SQ_INSTS_VALU_MFMA_MOPS_F64 -d {trace_dir} "
+           f"-o run --output-format csv -- python3 bench.py --pmc-pass gemm --m {m} --n {n}",
+           f"    python3 bench.py --summarize-mfma {trace_dir} --m {m} --n {n}",
+           "",
+           "The workload: the trailing updates C (m-r0 x n-r0, leading dimension of the SVD's "
+           "working copy) -= [U X] [Y V]^T with inner dimension 64 for r0 = 32, 64, ... "
+           "(svd_gpu_dgemm, the kernel the bidiagonalization calls), then one 8192^3 GEMM "
+           "for scale.  MFMA GFLOP = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (whole 128 x 128 "
+           "tiles, so above the algorithmic 2 m n k at ragged edges); % of peak against "
+           f"{peak} TFLOP/s; MfmaUtil from the counter definitions (MFMA-busy cycles over "
+           "GPU-busy cycles x SIMDs); GB/s = the update's algorithmic bytes (C read and "
+           "written, panels read once) / duration.  Durations are those of the counter "
+           "run (dispatches serialized).",
+           "",
+           "| dispatch | us | alg. GFLOP | MFMA GFLOP | TFLOP/s | % of FP64 peak | MfmaUtil % "
+           "| alg. GB/s |",
+           "|---|---|---|---|---|---|---|---|"]
+    for i in (0, len(upd) // 4, len(upd) // 2, 3 * len(upd) // 4, len(upd) - 1):
+        if i >= len(upd) or i >= len(shapes):
+            continue
+        rm, rn = shapes[i]
+        out.append(line(f"update r0={m - rm} ({rm}x{rn}x64)", upd[i], 2.0 * rm * rn * 64,
+                        8.0 * (2 * rm * rn + 64 * (rm + rn))))
+    tot_ns = sum(v["ns"] for v in upd)
+    tot_fl = sum(512.0 * v["SQ_INSTS_VALU_MFMA_MOPS_F64"] for v in upd)
+    alg_fl = sum(2.0 * rm * rn * 64 for rm, rn in shapes[:len(upd)])
+    alg_b = sum(8.0 * (2 * rm * rn + 64 * (rm + rn)) for rm, rn in shapes[:len(upd)])
+    util = sum(v["MfmaUtil"] * v["ns"] for v in upd) / tot_ns
+    out += ["", f"All {len(upd)} updates: {tot_ns / 1e6:.2f} ms, {alg_fl / 1e9:.1f} algorithmic "
+            f"GFLOP ({tot_fl / 1e9:.1f} on the MFMA units) -> {alg_fl / tot_ns / 1e3:.1f} TFLOP/s "
+            f"({100 * alg_fl / tot_ns / 1e3 / peak:.1f}% of FP64 peak), time-weighted MfmaUtil "
+            f"{util:.1f}%, {alg_b / tot_ns:.0f} GB/s algorithmic ({100 * alg_b / tot_ns / 8000:.0f}% "
+            "of 8 TB/s): the update (inner dimension 64) is bound by the trailing matrix's read "
+            "and write, not by the matrix cores.", ""]
+    out.append("| dispatch | us | alg. GFLOP | MFMA GFLOP | TFLOP/s | % of FP64 peak | MfmaUtil % "
+               "| alg. GB/s |")
+    out.append("|---|---|---|---|---|---|---|---|")
+    out.append(line("8192^3 GEMM", big, 2.0 * 8192 ** 3, 8.0 * 3 * 8192 ** 2))
+    path = os.path.join(ROOT, "profiles", f"k_dgemm_{m}x{n}_mfma.md")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    print("\n".join(out))
+    return 0
+
+
 def mixed_form(n: int = 8192, seed: int = 3) -> dict:
     """A = Q diag(sigma) P^T with Haar Q, P (QR of seeded N(0,1), columns
     signed by diag R) and sigma_i = 10^(-6 i/(n-1)); plain FP64 against the
@@ -592,6 +771,10 @@ def main() -> int:
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("KERNEL", "FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic summary of a GEMV kernel (left or right) from two "
                          "--pmc pass outputs")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_DIR", "BENCH_JSON"),
+                    help="write the kernel-time summary from a --kernel-trace run of the bench")
+    ap.add_argument("--summarize-mfma", metavar="TRACE_DIR",
+                    help="write the MFMA summary of the GEMM from a --pmc-pass gemm output")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -603,6 +786,10 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-forms", action="store_true")
     args = ap.parse_args()
+    if args.summarize_trace:
+        return summarize_trace(*args.summarize_trace, args.m, args.n)
+    if args.summarize_mfma:
+        return summarize_mfma(args.summarize_mfma, args.m, args.n)
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc, args.m, args.n)
     if args.pmc_pass:
