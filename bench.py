@@ -466,14 +466,15 @@ def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -
     return 0
 
 
-def summarize_trace(trace_dir: str, bench_json: str, m: int, n: int) -> int:
+def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n: int) -> int:
     """Reads the kernel trace of `rocprofv3 --kernel-trace --stats -- python3
     bench.py --steps 3 --warmup 1 --no-cpu --no-forms` (nine m x n SVDs: one
     warm-up, three timed, three kernel-timed, two through the host-buffer
     entry point) and the bench line that run printed, and writes
     profiles/svd_gpu_{m}x{n}_kernels.md: time per SVD by kernel, the
     bidiagonalization kernels by column-step range, and the dominant kernel's
-    average launch against the bench line's roofline figure."""
+    average launch against the roofline figure of the bench line of the same
+    command run without the profiler (plain_json)."""
     import csv
     import glob
     import re
@@ -519,10 +520,13 @@ def summarize_trace(trace_dir: str, bench_json: str, m: int, n: int) -> int:
            "`TMPDIR=/tmp`):", "",
            f"    rocprofv3 --kernel-trace --stats -d {trace_dir} -o run --output-format csv -- "
            f"python3 bench.py --steps 3 --warmup 1 --no-cpu --no-forms > {bench_json}",
-           f"    python3 bench.py --summarize-trace {trace_dir} {bench_json} --m {m} --n {n}", "",
+           f"    python3 bench.py --steps 3 --warmup 1 --no-cpu --no-forms > {plain_json}",
+           f"    python3 bench.py --summarize-trace {trace_dir} {bench_json} {plain_json} "
+           f"--m {m} --n {n}", "",
            f"All dispatches of the run ({len(rows)}; {nsvd} SVDs: one warm-up, three timed, "
            "three kernel-timed, two through the host-buffer entry point, whose staged copies "
-           "are the copy rows), divided by the number of SVDs.", "",
+           "are the copy rows; the residual checks' torch GEMMs are the large rocBLAS rows "
+           "under one call per SVD), divided by the number of SVDs.", "",
            "```",
            f"{'kernel':52s} {'calls/SVD':>9s} {'ms/SVD':>8s} {'avg us':>9s} {'%':>5s}"]
     for sn, (c, us) in sorted(agg.items(), key=lambda x: -x[1][1]):
@@ -542,17 +546,25 @@ def summarize_trace(trace_dir: str, bench_json: str, m: int, n: int) -> int:
         out.append(f"{lo:6d}-{hi:6d} " + " ".join(cells))
     out.append("```")
     try:
-        line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+        prof = json.loads(open(bench_json).read().strip().splitlines()[-1])["roofline"]
+        line = json.loads(open(plain_json).read().strip().splitlines()[-1])
         rf = line["roofline"]
         kern = rf["kernel"]
         c, us = agg.get(next(sn for sn in agg if kern in sn), (0, 0.0))
-        out += ["", f"Agreement with the bench line of the same run: `roofline.avg_launch_us` = "
-                f"{rf['avg_launch_us']:.2f} us for `{kern}` (HIP event pairs around every launch "
-                f"in the kernel-timed steps); this trace: {us / c:.2f} us per launch over all "
-                f"{c} launches.  Algorithmic bytes per launch "
+        out += ["", f"Agreement with the bench line: run without the profiler, the bench "
+                f"reports `roofline.avg_launch_us` = {rf['avg_launch_us']:.2f} us for `{kern}` "
+                f"(start and stop stamps of every launch in its kernel-timed steps, "
+                f"{line['ms_per_step']:.1f} ms per SVD in its timed steps); this trace gives "
+                f"{us / c:.2f} us per launch over all {c} launches.  Under the profiler the "
+                f"bench's own stamps read {prof['avg_launch_us']:.2f} us (the profiler's "
+                f"interception of the stamped launches), so `frac` is taken from the run "
+                f"without it.  Algorithmic bytes per launch "
                 f"{rf['algorithmic_bytes_per_launch'] / 1e6:.1f} MB; counter traffic "
-                f"{(rf['traffic'] or 0) / 1e6:.1f} MB (profiles/{kern}_{m}x{n}_traffic.json); "
-                f"achieved {rf['achieved']:.0f} GB/s = {rf['frac']:.3f} of 8 TB/s."]
+                f"{(traffic_summary(m, n, kern) or 0) / 1e6:.1f} MB "
+                f"(profiles/{kern}_{m}x{n}_traffic.json); "
+                f"achieved {rf['achieved']:.0f} GB/s = {rf['frac']:.3f} of 8 TB/s (from this "
+                f"trace's average: "
+                f"{rf['algorithmic_bytes_per_launch'] / (us / c) / 1e3:.0f} GB/s)."]
     except (OSError, ValueError, KeyError, StopIteration):
         pass
     path = os.path.join(ROOT, "profiles", f"svd_gpu_{m}x{n}_kernels.md")
@@ -771,7 +783,8 @@ def main() -> int:
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("KERNEL", "FETCH_DIR", "WRITE_DIR"),
                     help="write the traffic summary of a GEMV kernel (left or right) from two "
                          "--pmc pass outputs")
-    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_DIR", "BENCH_JSON"),
+    ap.add_argument("--summarize-trace", nargs=3,
+                    metavar=("TRACE_DIR", "BENCH_JSON", "PLAIN_BENCH_JSON"),
                     help="write the kernel-time summary from a --kernel-trace run of the bench")
     ap.add_argument("--summarize-mfma", metavar="TRACE_DIR",
                     help="write the MFMA summary of the GEMM from a --pmc-pass gemm output")
