@@ -275,6 +275,14 @@ def batched_form(seed: int = 0, count: int = 10000, reps: int = 3, loop_sample: 
             worst_res = max(worst_res, float(np.linalg.norm(a - (u * s) @ v.T)
                                              / np.linalg.norm(a)))
         inf = info.cpu().numpy()
+        # algorithmic work of one-sided Jacobi on the tall orientation (M x N):
+        # per sweep N(N-1)/2 rotations, each one cross dot product (2M flops;
+        # the column norms are carried through the rotations) and the
+        # rotation of two columns of the matrix (6M) and of V (6N)
+        big, small = np.maximum(ms, ns).astype(float), np.minimum(ms, ns).astype(float)
+        flops = float(np.sum(np.maximum(inf, 0) * small * (small - 1) / 2
+                             * (8 * big + 6 * small)))
+        peak = 78.6 if name == "fp64" else 157.3   # TFLOP/s, MI355X vector FP64 / FP32
         progress(f"batched {name}: {1e3 * best:.1f} ms per batch (first run with tuning "
                  f"{first:.1f} s)")
         res[name] = {"svd_per_s": count / best, "ms": 1e3 * best,
@@ -282,7 +290,13 @@ def batched_form(seed: int = 0, count: int = 10000, reps: int = 3, loop_sample: 
                      "first_run_s_incl_tuning": first, "tuned": b.config(),
                      "sweeps_mean": float(inf.mean()), "unconverged": int(np.sum(inf < 0)),
                      "checked_matrices": int(len(sample)),
-                     "worst_residual": worst_res, "worst_sigma_err_rel_max": worst_sig}
+                     "worst_residual": worst_res, "worst_sigma_err_rel_max": worst_sig,
+                     "roofline": {"bound": "valu", "achieved": flops / best / 1e12,
+                                  "peak": peak, "unit": "TFLOP/s",
+                                  "frac": flops / best / 1e12 / peak,
+                                  "flops_per_batch": flops,
+                                  "work": "sum over matrices of sweeps x N(N-1)/2 rotations "
+                                          "x (8M + 6N) flops, M x N the tall orientation"}}
         b.close()
         del A, S, U, V
     # loop of plain svd_gpu_dev calls over the first matrices of the batch

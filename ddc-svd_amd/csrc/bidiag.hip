@@ -633,6 +633,10 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   const int td = dotr ? tid : 0;
   const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
                               : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
+  // row k+1 of the panel's Y and V (to LDS below), with the batch too
+  const int tk = min(tid, t);
+  const double yk1v = a.Y[(k + 1) + (int64_t)tk * a.ldy];
+  const double vk1v = a.V[(k + 1) + (int64_t)tk * a.ldy];
   // the first PB chunk partials of the panel dots (Rdot) and of this row's
   // A0 r (Rpart), clamped to the last chunk, in the same batch as the loads
   // above: one wait for all of them
@@ -678,8 +682,8 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   if (!noright) R.make(wave_sum(rn), rk1);
   if (dotr) dyv[tid] = R.dot(ds, w1);
   double x = R.dot(h, c);  // A0[row, k+1:n] v, with c = A0[row, k+1]; 0 if noright
-  if (tid <= t) yk1[tid] = a.Y[(k + 1) + (int64_t)tid * a.ldy];
-  if (tid < t) vk1[tid] = a.V[(k + 1) + (int64_t)tid * a.ldy];
+  if (tid <= t) yk1[tid] = yk1v;
+  if (tid < t) vk1[tid] = vk1v;
   if (tid == t) vk1[t] = noright ? 0.0 : R.vec(rk1, true);
   __syncthreads();
 
