@@ -119,10 +119,37 @@ static int trsm_t(int M, int N, const float* T, int ldt, float* B, int ldb, hipS
   return strsm_left_upper(M, N, T, ldt, B, ldb, st);
 }
 
+// The left and right applications are independent: the right one runs on a
+// second stream of the device, forked after the unpacking and joined before
+// returning, so that the two sequences of block products (each ending in
+// small, latency-bound GEMMs and triangular solves) overlap.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kMaxSideDev = 64;
+static SideStream g_side[kMaxSideDev];
+
+static int side_stream(SideStream** out) {
+  int dev = 0;
+  DDC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxSideDev) return DDC_E_ARG;
+  SideStream& sd = g_side[dev];
+  if (!sd.s) DDC_HIP(hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking));
+  if (!sd.fork) DDC_HIP(hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming));
+  if (!sd.join) DDC_HIP(hipEventCreateWithFlags(&sd.join, hipEventDisableTiming));
+  *out = &sd;
+  return DDC_OK;
+}
+
 void backtrans_plan(ArenaPlan& p, int m, int n) {
   p.add<double>((size_t)NBB * NBB * 2);  // G, T^{-1}
   p.add<double>((size_t)NBB * n);        // Z
   p.add<double>((size_t)m * NBB);        // packed reflectors
+  // the right-hand application's own set
+  p.add<double>((size_t)NBB * NBB * 2);
+  p.add<double>((size_t)NBB * n);
+  p.add<double>((size_t)n * NBB);
 }
 
 // Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C:
@@ -162,10 +189,16 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
                   int ldv, Arena& ws, hipStream_t st) {
   if (m < n || n < 1) return DDC_E_ARG;
   double* G = ws.take<double>((size_t)NBB * NBB * 2);
-  double* Ti = G + NBB * NBB;
+  double* Ti = G ? G + NBB * NBB : nullptr;
   double* Z = ws.take<double>((size_t)NBB * n);
   double* Wd = ws.take<double>((size_t)m * NBB);
-  if (!Wd) return DDC_E_NOMEM;
+  double* G2 = ws.take<double>((size_t)NBB * NBB * 2);
+  double* Ti2 = G2 ? G2 + NBB * NBB : nullptr;
+  double* Z2 = ws.take<double>((size_t)NBB * n);
+  double* Wd2 = ws.take<double>((size_t)n * NBB);
+  if (!Wd2) return DDC_E_NOMEM;
+  SideStream* sd = nullptr;
+  DDC_TRY(side_stream(&sd));
   dim3 tb(32, 8);
   dim3 gu((unsigned)ceil_div(n, 32), (unsigned)ceil_div(m, 32));
   hipLaunchKernelGGL(bt_unpack<double>, gu, tb, 0, st, n, m, Ut, ldb, U, ldu);
@@ -173,17 +206,32 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   dim3 gv((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
   hipLaunchKernelGGL(bt_unpack<double>, gv, tb, 0, st, n, n, Vt, ldb, V, ldv);
   DDC_LAUNCHED();
+  const bool right = n > 2;
+  if (right) {
+    DDC_HIP(hipEventRecord(sd->fork, st));
+    DDC_HIP(hipStreamWaitEvent(sd->s, sd->fork, 0));
+  }
   // left: reflectors 0..n-1 (the last one is the 1 x 1 sign flip when m == n)
-  DDC_TRY(apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st));
-  // right: reflectors 0..n-3 (row n-2 holds no reflection)
-  if (n > 2) DDC_TRY(apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G, Ti, Z, Wd, st));
+  const int rcl = apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st);
+  // right: reflectors 0..n-3 (row n-2 holds no reflection); joined even on
+  // failure, so that nothing of this call still runs when it returns
+  int rcr = DDC_OK;
+  if (right) {
+    rcr = apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G2, Ti2, Z2, Wd2, sd->s);
+    DDC_HIP(hipEventRecord(sd->join, sd->s));
+    DDC_HIP(hipStreamWaitEvent(st, sd->join, 0));
+  }
+  DDC_TRY(rcl);
+  DDC_TRY(rcr);
   return DDC_OK;
 }
 
 void backtrans_plan_f32(ArenaPlan& p, int n) {
-  p.add<float>((size_t)NBB * NBB * 2);
-  p.add<float>((size_t)NBB * n);
-  p.add<float>((size_t)n * NBB);
+  for (int side = 0; side < 2; ++side) {
+    p.add<float>((size_t)NBB * NBB * 2);
+    p.add<float>((size_t)NBB * n);
+    p.add<float>((size_t)n * NBB);
+  }
 }
 
 // FP32 arithmetic throughout (the mixed-precision form refines the result).
@@ -194,14 +242,32 @@ int backtrans_run_f32(int n, const float* A32, int lda, const double* Ut, const 
   float* Ti = G ? G + NBB * NBB : nullptr;
   float* Z = ws.take<float>((size_t)NBB * n);
   float* Wd = ws.take<float>((size_t)n * NBB);
-  if (!Wd) return DDC_E_NOMEM;
+  float* G2 = ws.take<float>((size_t)NBB * NBB * 2);
+  float* Ti2 = G2 ? G2 + NBB * NBB : nullptr;
+  float* Z2 = ws.take<float>((size_t)NBB * n);
+  float* Wd2 = ws.take<float>((size_t)n * NBB);
+  if (!Wd2) return DDC_E_NOMEM;
+  SideStream* sd = nullptr;
+  DDC_TRY(side_stream(&sd));
   dim3 tb(32, 8);
   dim3 g((unsigned)ceil_div(n, 32), (unsigned)ceil_div(n, 32));
   hipLaunchKernelGGL(bt_unpack<float>, g, tb, 0, st, n, n, Ut, ldb, U32, n);
   hipLaunchKernelGGL(bt_unpack<float>, g, tb, 0, st, n, n, Vt, ldb, V32, n);
   DDC_LAUNCHED();
-  DDC_TRY(apply_blocks<float>(true, n, n, A32, lda, n, U32, n, n, G, Ti, Z, Wd, st));
-  if (n > 2) DDC_TRY(apply_blocks<float>(false, n, n, A32, lda, n - 2, V32, n, n, G, Ti, Z, Wd, st));
+  const bool right = n > 2;
+  if (right) {
+    DDC_HIP(hipEventRecord(sd->fork, st));
+    DDC_HIP(hipStreamWaitEvent(sd->s, sd->fork, 0));
+  }
+  const int rcl = apply_blocks<float>(true, n, n, A32, lda, n, U32, n, n, G, Ti, Z, Wd, st);
+  int rcr = DDC_OK;
+  if (right) {
+    rcr = apply_blocks<float>(false, n, n, A32, lda, n - 2, V32, n, n, G2, Ti2, Z2, Wd2, sd->s);
+    DDC_HIP(hipEventRecord(sd->join, sd->s));
+    DDC_HIP(hipStreamWaitEvent(st, sd->join, 0));
+  }
+  DDC_TRY(rcl);
+  DDC_TRY(rcr);
   return DDC_OK;
 }
 

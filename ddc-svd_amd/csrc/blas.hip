@@ -14,7 +14,13 @@ namespace ddc {
 namespace {
 std::mutex g_blas_mu;
 constexpr int kMaxDev = 64;
-rocblas_handle g_handle[kMaxDev] = {};
+// One handle per (device, stream) for up to kSlots streams: a handle's
+// workspace (split-K GEMMs, the triangular solve) must not be shared by
+// calls that may run concurrently on different streams.  Further streams
+// share slot 0, re-bound per call.
+constexpr int kSlots = 4;
+rocblas_handle g_handle[kMaxDev][kSlots] = {};
+hipStream_t g_hstream[kMaxDev][kSlots] = {};
 
 rocblas_operation op_of(char t) {
   return (t == 'T' || t == 't') ? rocblas_operation_transpose : rocblas_operation_none;
@@ -33,16 +39,25 @@ static int blas_handle(hipStream_t st, rocblas_handle* h) {
   int dev = 0;
   DDC_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDev) return DDC_E_ARG;
-  if (!g_handle[dev]) {
-    rocblas_status s = rocblas_create_handle(&g_handle[dev]);
-    if (s != rocblas_status_success) {
-      g_handle[dev] = nullptr;
-      return blas_fail("rocblas_create_handle", s);
+  int slot = -1;
+  for (int i = 0; i < kSlots && slot < 0; ++i)
+    if (g_handle[dev][i] && g_hstream[dev][i] == st) slot = i;
+  for (int i = 0; i < kSlots && slot < 0; ++i)
+    if (!g_handle[dev][i]) {
+      rocblas_status s = rocblas_create_handle(&g_handle[dev][i]);
+      if (s != rocblas_status_success) {
+        g_handle[dev][i] = nullptr;
+        return blas_fail("rocblas_create_handle", s);
+      }
+      slot = i;
     }
+  if (slot < 0) slot = 0;
+  if (g_hstream[dev][slot] != st || slot == 0) {
+    rocblas_status s = rocblas_set_stream(g_handle[dev][slot], st);
+    if (s != rocblas_status_success) return blas_fail("rocblas_set_stream", s);
+    g_hstream[dev][slot] = st;
   }
-  rocblas_status s = rocblas_set_stream(g_handle[dev], st);
-  if (s != rocblas_status_success) return blas_fail("rocblas_set_stream", s);
-  *h = g_handle[dev];
+  *h = g_handle[dev][slot];
   return DDC_OK;
 }
 
