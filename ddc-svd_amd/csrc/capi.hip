@@ -214,24 +214,33 @@ static void host_copy2d(char* dst, size_t dpitch, const char* src, size_t spitch
   }
   constexpr int NT = 4;
   std::thread th[NT - 1];
-  if (contiguous) {
-    // split the bytes, whatever the column count
-    const size_t per = (bytes + NT - 1) / NT;
-    for (int t = 1; t < NT; ++t) {
-      const size_t b0 = std::min(bytes, t * per), b1 = std::min(bytes, (t + 1) * per);
-      th[t - 1] = std::thread([=] { memcpy(dst + b0, src + b0, b1 - b0); });
+  // parts [t * per, (t + 1) * per) of the bytes (contiguous) or of the
+  // columns; a part whose thread cannot be started is copied here
+  const size_t units = contiguous ? bytes : cols;
+  const size_t per = (units + NT - 1) / NT;
+  auto part = [=](int t) {
+    const size_t u0 = std::min(units, t * per), u1 = std::min(units, (t + 1) * per);
+    if (contiguous)
+      memcpy(dst + u0, src + u0, u1 - u0);
+    else
+      run(u0, u1);
+  };
+  bool started[NT - 1] = {};
+  for (int t = 1; t < NT; ++t) {
+    try {
+      th[t - 1] = std::thread(part, t);
+      started[t - 1] = true;
+    } catch (...) {
+      started[t - 1] = false;
     }
-    memcpy(dst, src, std::min(bytes, per));
-  } else {
-    const size_t per = (cols + NT - 1) / NT;
-    for (int t = 1; t < NT; ++t) {
-      const size_t c0 = std::min(cols, t * per), c1 = std::min(cols, (t + 1) * per);
-      th[t - 1] = std::thread(run, c0, c1);
-    }
-    run(0, std::min(cols, per));
   }
-  for (auto& t : th)
-    if (t.joinable()) t.join();
+  part(0);
+  for (int t = 1; t < NT; ++t) {
+    if (started[t - 1])
+      th[t - 1].join();
+    else
+      part(t);
+  }
 }
 
 // host (pageable) -> device, stream-ordered on st; the caller synchronizes

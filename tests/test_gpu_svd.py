@@ -448,6 +448,29 @@ def test_profile_right_gemv_hook(ddcsvd):
                                                                    device="cuda")), 1)
 
 
+@pytest.mark.parametrize("shape", [(4100, 1100), (1100, 4100)])
+def test_host_buffers_staged_copies_match_device_path(ddcsvd, shape):
+    """Host-buffer calls above 4 MiB move their data through the library's
+    pinned staging buffers in 32 MiB slices (36 MB here: two slices, and a
+    wide input whose A goes back through the transposed layout).  The
+    results must equal the device-pointer path's bit for bit (same kernels),
+    and the overwritten A the reflectors of bidiag_gpu, whose copies are not
+    staged."""
+    import torch
+    m, n = shape
+    A = gen.normal_matrix(m, n, 41)
+    gpu_case(f"svd_gpu host buffers staged {m}x{n}")
+    s, U, V, A_after = ddcsvd.svd(A)
+    At = ddcsvd.col_major(torch.from_numpy(A).to("cuda"))
+    sd, Ud, Vd = ddcsvd.svd_torch(At)
+    torch.cuda.synchronize()
+    assert np.array_equal(s, sd.cpu().numpy())
+    assert np.array_equal(U, Ud.cpu().numpy())
+    assert np.array_equal(V, Vd.cpu().numpy())
+    Ab, _, _ = ddcsvd.bidiag(A)
+    assert np.array_equal(A_after, Ab)
+
+
 def test_svd_dev_null_stream_orders_after_torch_work(ddcsvd):
     """svd_gpu_dev with stream NULL (torch's default stream handle is 0) must
     run after the caller's pending work on that stream: A is produced by a
