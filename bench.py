@@ -365,7 +365,9 @@ def pmc_pass(kernel: str, m: int, n: int, seed: int) -> int:
     trailing-update GEMMs of one m x n bidiagonalization, C (m-r0 x n-r0,
     ld of the SVD's working copy) -= [U X] [Y V]^T with inner dimension 64,
     r0 = 32, 64, ..., through svd_gpu_dgemm (the hand-written MFMA kernel),
-    and one 8192^3 GEMM."""
+    and one 8192^3 GEMM.  kernel "backtrans": the back-transformation's
+    compact-WY blocks at the m x n shapes (bidiag_form_u_gpu and
+    bidiag_form_v_gpu on synthetic unit-norm reflectors)."""
     import numpy as np
     import torch
 
@@ -373,6 +375,25 @@ def pmc_pass(kernel: str, m: int, n: int, seed: int) -> int:
     from ddcsvd import gen
 
     ddcsvd.load()
+    if kernel == "backtrans":
+        # reflector storage of an m x n bidiagonalization: unit-norm left
+        # reflectors A[k:m, k] and right reflectors A[k, k+1:n] (random
+        # directions), applied to the identity by bidiag_form_u_gpu /
+        # bidiag_form_v_gpu: the back-transformation's block products at the
+        # SVD's shapes
+        A = gen.normal_matrix(m, n, seed)
+        lo = np.tril(A)
+        lo /= np.linalg.norm(lo, axis=0, keepdims=True)
+        up = np.triu(A, 1)
+        rn = np.linalg.norm(up, axis=1, keepdims=True)
+        up /= np.where(rn > 0, rn, 1.0)
+        A = np.asfortranarray(lo + up)
+        del lo, up
+        print(f"pmc pass: back-transformation blocks of {m}x{n} (form U, form V)", flush=True)
+        ddcsvd.form_u(A)
+        ddcsvd.form_v(A)
+        print("pmc pass done", flush=True)
+        return 0
     if kernel == "gemm":
         lda = (m + 15) // 16 * 16
         lda = lda + 8 if lda % 256 == 0 else lda   # the SVD's working-copy layout
@@ -477,6 +498,72 @@ def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
+    return 0
+
+
+def summarize_mfma_backtrans(trace_dir: str, m: int, n: int) -> int:
+    """Reads the counter CSV of a --pmc-pass backtrans run (MfmaUtil and
+    SQ_INSTS_VALU_MFMA_MOPS_F64 with the kernel trace) and writes
+    profiles/backtrans_{m}x{n}_mfma.md: the compact-WY block products
+    (rocBLAS FP64 GEMM and triangular-solve kernels) grouped by kernel, MFMA
+    FLOPs = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64, durations from the dispatch
+    timestamps, time-weighted MfmaUtil."""
+    import csv
+    import glob
+    import re
+
+    files = glob.glob(os.path.join(trace_dir, "**", "*counter_collection*.csv"), recursive=True)
+    per = {}
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            d = per.setdefault((f, int(r["Dispatch_Id"])), {})
+            d[r["Counter_Name"]] = float(r["Counter_Value"])
+            d["name"] = r["Kernel_Name"]
+            d["ns"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    groups = {}
+    for v in per.values():
+        nm = v["name"]
+        if nm.startswith("__amd_rocclr"):   # the entry points' host copies
+            continue
+        mt = re.search(r"MT(\d+x\d+x\d+)", nm)
+        key = (f"rocBLAS GEMM {mt.group(1)}" if mt else
+               re.sub(r"[<(].*", "", nm.replace("void ", "")).strip()[:48])
+        g = groups.setdefault(key, [0, 0.0, 0.0, 0.0])
+        g[0] += 1
+        g[1] += v["ns"]
+        g[2] += 512.0 * v.get("SQ_INSTS_VALU_MFMA_MOPS_F64", 0.0)
+        g[3] += v.get("MfmaUtil", 0.0) * v["ns"]
+    peak = 78.6
+    out = [f"# Back-transformation block products at {m}x{n}: MFMA utilisation", "",
+           "Command (MI355X, ROCm 7.2, one GPU; run from the repository root with "
+           "`TMPDIR=/tmp`):", "",
+           f"    rocprofv3 --kernel-trace --pmc MfmaUtil SQ_INSTS_VALU_MFMA_MOPS_F64 -d {trace_dir} "
+           f"-o run --output-format csv -- python3 bench.py --pmc-pass backtrans --m {m} --n {n}",
+           f"    python3 bench.py --summarize-mfma-backtrans {trace_dir} --m {m} --n {n}", "",
+           "The workload: unit-norm reflectors in the storage of an m x n bidiagonalization, "
+           "applied to the identity by bidiag_form_u_gpu and bidiag_form_v_gpu, which run the "
+           "back-transformation's compact-WY blocks (512 reflectors: G = W^T W, Z = W^T C, "
+           "triangular solve, C -= W Z) at the SVD's shapes; the entry points' host copies "
+           "are left out.  Durations are those of the "
+           "counter run (dispatches serialized); MfmaUtil is MFMA-busy cycles over GPU-busy "
+           f"cycles x SIMDs; % of peak against {peak} TFLOP/s FP64.", "",
+           "| kernel | dispatches | ms | MFMA GFLOP | TFLOP/s | % of FP64 peak | MfmaUtil % |",
+           "|---|---|---|---|---|---|---|"]
+    tot = [0, 0.0, 0.0, 0.0]
+    for key, (c, ns, fl, ut) in sorted(groups.items(), key=lambda x: -x[1][1]):
+        if fl <= 0 and ns < 1e5:
+            continue
+        out.append(f"| {key} | {c} | {ns / 1e6:.2f} | {fl / 1e9:.1f} | {fl / ns / 1e3:.1f} | "
+                   f"{100 * fl / ns / 1e3 / peak:.1f} | {ut / ns:.1f} |")
+    for c, ns, fl, ut in groups.values():
+        tot = [tot[0] + c, tot[1] + ns, tot[2] + fl, tot[3] + ut]
+    out += ["", f"All {tot[0]} dispatches: {tot[1] / 1e6:.2f} ms, {tot[2] / 1e9:.1f} MFMA GFLOP, "
+            f"{tot[2] / tot[1] / 1e3:.1f} TFLOP/s ({100 * tot[2] / tot[1] / 1e3 / peak:.1f}% of "
+            f"FP64 peak), time-weighted MfmaUtil {tot[3] / tot[1]:.1f}%."]
+    path = os.path.join(ROOT, "profiles", f"backtrans_{m}x{n}_mfma.md")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    print("\n".join(out))
     return 0
 
 
@@ -791,7 +878,7 @@ def mixed_gaussian(n: int = 8192, seed: int = 2) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pmc-pass", choices=["left", "right", "gemm"],
+    ap.add_argument("--pmc-pass", choices=["left", "right", "gemm", "backtrans"],
                     help="workload of one rocprofv3 --pmc pass (copy calibration + one "
                          "GEMV kernel's launches, or the trailing-update GEMMs)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("KERNEL", "FETCH_DIR", "WRITE_DIR"),
@@ -800,6 +887,9 @@ def main() -> int:
     ap.add_argument("--summarize-trace", nargs=3,
                     metavar=("TRACE_DIR", "BENCH_JSON", "PLAIN_BENCH_JSON"),
                     help="write the kernel-time summary from a --kernel-trace run of the bench")
+    ap.add_argument("--summarize-mfma-backtrans", metavar="TRACE_DIR",
+                    help="write the MFMA summary of the back-transformation blocks from a "
+                         "--pmc-pass backtrans output")
     ap.add_argument("--summarize-mfma", metavar="TRACE_DIR",
                     help="write the MFMA summary of the GEMM from a --pmc-pass gemm output")
     ap.add_argument("--gpus", type=int, default=1)
@@ -815,6 +905,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.summarize_trace:
         return summarize_trace(*args.summarize_trace, args.m, args.n)
+    if args.summarize_mfma_backtrans:
+        return summarize_mfma_backtrans(args.summarize_mfma_backtrans, args.m, args.n)
     if args.summarize_mfma:
         return summarize_mfma(args.summarize_mfma, args.m, args.n)
     if args.summarize_pmc:
