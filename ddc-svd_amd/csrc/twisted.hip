@@ -57,6 +57,7 @@ constexpr double TW_GTOL = 1e-6;   // relative gap (in sigma^2) that forms a gro
 constexpr int TW_KMAX = 64;        // members per Rayleigh-Ritz window
 constexpr int TW_CK = 64;          // checkpoint interval = LDS chunk
 constexpr int TW_MAXBIS = 200;     // bisection steps per lane at most
+constexpr double TW_CONV = 32.0;   // Rayleigh correction accepted below TW_CONV eps lam
 
 enum { TW_UNRES = 1 };
 
@@ -603,11 +604,13 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
     }
     if (!active) continue;
     corr = gbest / nz2;
-    // converged: correction at the rounding level of lam, the vector's norm
-    // saturated, or (from the third step on) the correction no longer
-    // halving -- it has reached the rounding floor of gamma_r, which for
-    // small singular values sits above eps*lam
-    if (fabs(corr) <= 4.0 * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS) ||
+    // converged: correction within a few ulps of lam (the shift's error is
+    // then far below any gap the vector can resolve, and the value returned
+    // includes the correction), the vector's norm saturated, or (from the
+    // third step on) the correction no longer halving -- it has reached the
+    // rounding floor of gamma_r, which for small singular values sits above
+    // eps*lam
+    if (fabs(corr) <= TW_CONV * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS) ||
         (it >= 2 && fabs(corr) > 0.5 * prev)) {
       active = false;
       continue;
