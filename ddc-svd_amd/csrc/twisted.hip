@@ -64,13 +64,25 @@ enum { TW_UNRES = 1 };
 __device__ __forceinline__ double guard(double x, double pivmin) {
   return (fabs(x) < pivmin) ? -pivmin : x;
 }
+// The quotient of a step is formed whether or not the step is a block
+// boundary, and the boundary value selected afterwards: with no branch around
+// the division, the operand loads of a run of steps can all be issued ahead
+// of the dependent arithmetic (the empty asm, opaque to the optimiser but no
+// scheduling barrier, keeps the compiler from moving the quotient back under
+// a branch).  The selected values are those of the plain expressions.
+__device__ __forceinline__ double keep(double x) {
+  asm("" : "+v"(x));
+  return x;
+}
 // stationary qd step: s_{i+1} from s_i (lld_i = 0 is a block boundary)
 __device__ __forceinline__ double stat_next(double s, double dp, double lld, double lam) {
-  return lld == 0.0 ? -lam : lld * s / dp - lam;
+  const double q = keep(lld * s / dp);
+  return lld == 0.0 ? -lam : q - lam;
 }
 // progressive qd step: p_i from p_{i+1}
 __device__ __forceinline__ double prog_prev(double p, double d, double dm, double lld, double lam) {
-  return lld == 0.0 ? d - lam : d * p / dm - lam;
+  const double q = keep(d * p / dm);
+  return lld == 0.0 ? d - lam : q - lam;
 }
 
 // qd data for B and its reversal, with negligible super-diagonal entries set
@@ -391,7 +403,10 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
   const int n = a.n;
   const int nbr = (n + 63) / 64;
   const int rep = ((int)blockIdx.x >= nbr) ? 1 : 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // wave is uniform over the wave: chunk indices and lengths stay scalar,
+  // so the full-chunk loops below are straight-line code
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int idx = ((int)blockIdx.x - rep * nbr) * 64 + lane;
   const bool live = idx < n;
   const int li = live ? idx : 0;
@@ -448,21 +463,29 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
         const int c0 = cw * TW_CK, len = min(TW_CK, n - c0);
         if (wave == 0) {
           if (active) Sck[(int64_t)cw * T] = s;
+          auto step = [&](int e) {
+            const double dp = guard(qd_d[e] + s, pivmin);
+            cnt += (dp < 0.0);
+            s = stat_next(s, dp, qd_lld[e], lam);  // past the last index: unused
+          };
+          if (len == TW_CK) {
 #pragma unroll 8
-          for (int e = 0; e < TW_CK; ++e) {
-            if (e < len) {
-              const double dp = guard(qd_d[e] + s, pivmin);
-              cnt += (dp < 0.0);
-              s = stat_next(s, dp, qd_lld[e], lam);  // past the last index: unused
-            }
+            for (int e = 0; e < TW_CK; ++e) step(e);
+          } else {
+            for (int e = 0; e < len; ++e) step(e);
           }
         } else {
+          auto step = [&](int e) {
+            const double dm = guard(qd_lld[e] + p, pivmin);
+            p = prog_prev(p, qd_d[e], dm, qd_lld[e], lam);
+          };
+          // the chunk's steps e < len with c0 + e < n - 1, descending
+          const int etop = min(len, n - 1 - c0) - 1;
+          if (etop == TW_CK - 1) {
 #pragma unroll 8
-          for (int e = TW_CK - 1; e >= 0; --e) {
-            if (e < len && c0 + e < n - 1) {
-              const double dm = guard(qd_lld[e] + p, pivmin);
-              p = prog_prev(p, qd_d[e], dm, qd_lld[e], lam);
-            }
+            for (int e = TW_CK - 1; e >= 0; --e) step(e);
+          } else {
+            for (int e = etop; e >= 0; --e) step(e);
           }
           if (active) Pck[(int64_t)cw * T] = p;  // p at the chunk's first index
         }
@@ -476,26 +499,35 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
       const int c0 = c * TW_CK, len = min(TW_CK, n - c0);
       if (wave == 0) {
         double ss = Sck[(int64_t)c * T];
+        auto step = [&](int e) {
+          bw[e][lane] = ss;
+          const double dp = guard(qd_d[e] + ss, pivmin);
+          ss = stat_next(ss, dp, qd_lld[e], lam);
+        };
+        if (len == TW_CK) {
 #pragma unroll 8
-        for (int e = 0; e < TW_CK; ++e) {
-          if (e < len) {
-            bw[e][lane] = ss;
-            const double dp = guard(qd_d[e] + ss, pivmin);
-            ss = stat_next(ss, dp, qd_lld[e], lam);
-          }
+          for (int e = 0; e < TW_CK; ++e) step(e);
+        } else {
+          for (int e = 0; e < len; ++e) step(e);
         }
       } else {
         double pp = (c + 1 < nck) ? Pck[(int64_t)(c + 1) * T] : 0.0;  // p_{c0+len}
+        auto step = [&](int e) {
+          const double dm = guard(qd_lld[e] + pp, pivmin);
+          pp = prog_prev(pp, qd_d[e], dm, qd_lld[e], lam);
+          bw[e][lane] = pp;
+        };
+        if (len == TW_CK && c0 + TW_CK - 1 < n - 1) {
 #pragma unroll 8
-        for (int e = TW_CK - 1; e >= 0; --e) {
-          if (e < len) {
+          for (int e = TW_CK - 1; e >= 0; --e) step(e);
+        } else {
+          for (int e = len - 1; e >= 0; --e) {
             if (c0 + e == n - 1) {
               pp = dn1 - lam;
+              bw[e][lane] = pp;
             } else {
-              const double dm = guard(qd_lld[e] + pp, pivmin);
-              pp = prog_prev(pp, qd_d[e], dm, qd_lld[e], lam);
+              step(e);
             }
-            bw[e][lane] = pp;
           }
         }
       }
@@ -556,41 +588,55 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
             pp = dn1 - lam;  // p_{n-1}
             etop = len - 2;
           }
+          auto pstep = [&](int e) {
+            bw[e][lane] = pp;  // p_{c0+e+1}
+            const double dm = guard(qd_lld[e] + pp, pivmin);
+            pp = prog_prev(pp, qd_d[e], dm, qd_lld[e], lam);
+          };
+          if (etop == TW_CK - 1) {
 #pragma unroll 8
-          for (int e = TW_CK - 1; e >= 0; --e) {
-            if (e <= etop) {
-              bw[e][lane] = pp;  // p_{c0+e+1}
-              const double dm = guard(qd_lld[e] + pp, pivmin);
-              pp = prog_prev(pp, qd_d[e], dm, qd_lld[e], lam);
-            }
+            for (int e = TW_CK - 1; e >= 0; --e) pstep(e);
+          } else {
+            for (int e = etop; e >= 0; --e) pstep(e);
           }
+          // z in straight-line code (the quotient formed on every step and
+          // kept where the step applies), staged in bw and stored after
 #pragma unroll 8
           for (int e = 0; e < TW_CK; ++e) {
+            const bool go = e <= etop && active && c0 + e >= r;
+            const double zn = keep(-(qd_ld[e] / guard(qd_lld[e] + bw[e][lane], pivmin)) * z);
+            z = go ? zn : z;
+            nzp += go ? zn * zn : 0.0;
+            bw[e][lane] = zn;
+          }
+          for (int e = 0; e < TW_CK; ++e) {
             const int i = c0 + e;
-            if (e <= etop && active && i >= r) {
-              z = -(qd_ld[e] / guard(qd_lld[e] + bw[e][lane], pivmin)) * z;
-              nzp += z * z;
-              out[oidx(i + 1)] = z;
-            }
+            if (e <= etop && active && i >= r) out[oidx(i + 1)] = bw[e][lane];
           }
         } else {
           double ss = Sck[(int64_t)cw * T];
+          auto sstep = [&](int e) {
+            bw[e][lane] = ss;
+            const double dp = guard(qd_d[e] + ss, pivmin);
+            ss = stat_next(ss, dp, qd_lld[e], lam);
+          };
+          if (len == TW_CK) {
 #pragma unroll 8
-          for (int e = 0; e < TW_CK; ++e) {
-            if (e < len) {
-              bw[e][lane] = ss;
-              const double dp = guard(qd_d[e] + ss, pivmin);
-              ss = stat_next(ss, dp, qd_lld[e], lam);
-            }
+            for (int e = 0; e < TW_CK; ++e) sstep(e);
+          } else {
+            for (int e = 0; e < len; ++e) sstep(e);
           }
 #pragma unroll 8
           for (int e = TW_CK - 1; e >= 0; --e) {
+            const bool go = e < len && active && c0 + e < r;
+            const double zn = keep(-(qd_ld[e] / guard(qd_d[e] + bw[e][lane], pivmin)) * z);
+            z = go ? zn : z;
+            nzp += go ? zn * zn : 0.0;
+            bw[e][lane] = zn;
+          }
+          for (int e = TW_CK - 1; e >= 0; --e) {
             const int i = c0 + e;
-            if (e < len && active && i < r) {
-              z = -(qd_ld[e] / guard(qd_d[e] + bw[e][lane], pivmin)) * z;
-              nzp += z * z;
-              out[oidx(i)] = z;
-            }
+            if (e < len && active && i < r) out[oidx(i)] = bw[e][lane];
           }
         }
       }
