@@ -333,6 +333,23 @@ __device__ __forceinline__ double secular_wave(const double* d, const double* z,
   return 1.0 + wave_sum(f);
 }
 
+// The same with its derivative in tau: with g_j = (d_j - w)(d_j + w),
+// df/dtau = 2 w sum z_j^2 / g_j^2.
+__device__ __forceinline__ double2 secular_wave_d(const double* d, const double* z, int K, int o,
+                                                  double tau, int lane) {
+  const double dO = d[o];
+  double f = 0.0, fp = 0.0;
+  for (int j = lane; j < K; j += 64) {
+    const double zj = z[j];
+    const double dj = d[j];
+    const double g = ((dj - dO) - tau) * (dj + dO + tau);
+    const double t = (zj * zj) / g;
+    f += t;
+    fp += t / g;
+  }
+  return make_double2(1.0 + wave_sum(f), 2.0 * (dO + tau) * wave_sum(fp));
+}
+
 __device__ __forceinline__ double wave_prod(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v *= __shfl_xor(v, o, 64);
@@ -385,23 +402,39 @@ __global__ __launch_bounds__(256) void dc_roots(DcBufs B, int ibase) {
     A = st.znorm;
     sgn = 1.0;
   }
-  // bisection on |tau| in (lo, hi]; f is increasing in tau
-  double lo = 0.0, hi = A;
+  // Safeguarded Newton on |tau| = u in the bracket (lo, hi]: f is increasing
+  // in tau, and every evaluation narrows the bracket by its sign.  A Newton
+  // iterate inside the bracket is the next point; otherwise the bracket is
+  // bisected (geometrically while it spans a factor above four, as the
+  // roots lie at any relative offset from the pole).  Stops when the Newton
+  // step or the bracket is at the rounding level of u.
+  double lo = 0.0, hi = A, u = -1.0, res = -1.0;
   for (int itn = 0; itn < 400; ++itn) {
     double t;
-    if (lo == 0.0) t = hi * 0.0625;
+    if (u > lo && u < hi) t = u;
+    else if (lo == 0.0) t = hi * 0.0625;
     else if (hi > 4.0 * lo) t = sqrt(lo * hi);
     else t = 0.5 * (lo + hi);
     if (!(t > lo && t < hi)) break;
-    double f = secular_wave(d, z, K, o, sgn * t, lane);
+    const double2 f = secular_wave_d(d, z, K, o, sgn * t, lane);
+    if (f.x == 0.0) {
+      res = t;
+      break;
+    }
     // origin below the root: f < 0 -> root further out; origin above: f > 0
-    bool further = (sgn > 0.0) ? (f < 0.0) : (f > 0.0);
+    const bool further = (sgn > 0.0) ? (f.x < 0.0) : (f.x > 0.0);
     if (further) lo = t; else hi = t;
     if (hi - lo <= 2.0 * EPS * lo) break;
+    const double du = sgn * f.y;  // df/du
+    u = (du != 0.0 && fabs(du) < INFINITY) ? t - f.x / du : -1.0;
+    if (fabs(u - t) <= 2.0 * EPS * t && u > lo && u < hi) {
+      res = u;
+      break;
+    }
   }
   if (lane == 0) {
     B.org[M.off + i] = o;
-    B.tau[M.off + i] = sgn * 0.5 * (lo + hi);
+    B.tau[M.off + i] = sgn * (res > 0.0 ? res : 0.5 * (lo + hi));
   }
 }
 
