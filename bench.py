@@ -174,9 +174,14 @@ def cpu_lapack(m: int, n: int, seed: int) -> dict:
     }
 
 
-def left_gemv_bytes(m: int, n: int) -> int:
-    """Algorithmic bytes of all left GEMV launches: sum_{k<n-1} 8 (m-k)(n-k-1)."""
-    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 1))
+def left_gemv_bytes(m: int, n: int, fused: bool = True) -> int:
+    """Algorithmic bytes of all left GEMV launches: sum_{k<n-1} 8 (m-k)(n-k-1)
+    over the steps that launch one (m >= n).  fused: in the SVD the first
+    column of every panel after the first (k = 32, 64, ...) takes its partial
+    sums from the trailing update instead (the profiling hook launches every
+    step)."""
+    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 1)
+               if not fused or k == 0 or k % 32 != 0)
 
 
 def right_gemv_bytes(m: int, n: int) -> int:
@@ -466,7 +471,8 @@ def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -
         return per, names
 
     launches = n - 1 if kernel == "left" else n - 2
-    alg = (left_gemv_bytes(m, n) if kernel == "left" else right_gemv_bytes(m, n)) / launches
+    alg = (left_gemv_bytes(m, n, fused=False) if kernel == "left"
+           else right_gemv_bytes(m, n)) / launches
     out = {"workload": f"the {launches} {kernel}-GEMV launches of one {m}x{n} N(0,1) "
                        f"bidiagonalization (svd_gpu_profile_{kernel}_gemv), seed 2",
            "commands": [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run --output-format csv "
@@ -608,7 +614,7 @@ def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n:
         a[1] += (t1 - t0) / 1e3
         if "bd_init" in nm:
             k = -1
-        if "bd_left_gemv" in nm:
+        if "bd_left_gemv" in nm or "bd_left_red" in nm:
             k += 1
         for kern in ("bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
             if kern in nm and k >= 0:

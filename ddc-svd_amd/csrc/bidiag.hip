@@ -27,7 +27,9 @@
 // reproducible).  The two GEMVs are the only passes over the trailing
 // matrix: 16 bytes per trailing element per step.  After
 // NB steps the trailing matrix gets A0 -= [U X] [Y V]^T, one FP64 MFMA GEMM
-// of inner dimension 2 NB (rocBLAS), one read and one write.
+// of inner dimension 2 NB (gemm.hip, one read and one write), which also
+// takes the partial sums of the next panel's first left GEMV while each
+// tile is in registers, so that step launches no left GEMV.
 #include <algorithm>
 
 #include "ddc_common.h"
@@ -281,6 +283,41 @@ __global__ __launch_bounds__(256) void bd_left_gemv(StepArgs<TS> a, int ncb, int
       a.Ldot[(int64_t)blockIdx.y * (2 * NB) + q0 + tid] =
           red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
   }
+}
+
+// The left GEMV of the first column k of a panel when the trailing update
+// has already formed its partial sums, one per 128-row tile (nt of them, in
+// Lpart rows 0..nt-1): blocks [0, ncs) sum them in tile order into Lpart
+// row 0 for the columns k+1..n-1; the other blocks store the reflector of
+// column k (rows k..m-1) in A and Ucopy and its diagonal, as bd_left_gemv
+// does.
+template <typename TS>
+__global__ __launch_bounds__(256) void bd_left_red(StepArgs<TS> a, int nt, int ncs) {
+  __shared__ double s_scal;
+  const int tid = threadIdx.x;
+  const int k = a.k;
+  if ((int)blockIdx.x < ncs) {
+    const int j = k + 1 + blockIdx.x * 256 + tid;
+    if (j < a.n) {
+      const double g = sum_strided<32>(a.Lpart + j, nt, a.ldy);
+      a.Lpart[j] = g;
+    }
+    return;
+  }
+  if (tid < 64) {
+    const double s = sum_partials(a.cnorm, a.ncn);
+    if (tid == 0) s_scal = s;
+  }
+  __syncthreads();
+  Refl R;
+  R.make(s_scal, a.cvec[k]);
+  const int r = k + ((int)blockIdx.x - ncs) * 256 + tid;
+  if (r < a.m) {
+    const double u = R.vec(a.cvec[r], r == k);
+    a.A[r + (int64_t)k * a.lda] = (TS)u;
+    if (a.Ucopy) a.Ucopy[r + (int64_t)a.t * a.lda] = u;
+  }
+  if ((int)blockIdx.x == ncs && tid == 0) a.alpha[k] = R.diag();
 }
 
 // left_fin: one thread per trailing column j in [k+1, n), one wave per group.
@@ -800,7 +837,9 @@ struct BidiagBufs {
 static BidiagBufs bidiag_dims(int m, int n, int lda) {
   BidiagBufs d;
   d.ldy = (int)round_up(n, 16);
-  d.nchL_max = (int)ceil_div(lda, RC) + 1;
+  // left GEMV partials: one per 2048-row chunk, or one per 128-row tile of
+  // the trailing update that carries the next column's left GEMV
+  d.nchL_max = (int)std::max(ceil_div(lda, RC), ceil_div(lda, TRAIL_TILE)) + 1;
   d.nccR_max = (int)ceil_div(n, CCR) + 1;
   d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(std::max(m, n), FIN)) + 1;
   d.nrn_max = (int)ceil_div(n, FIN) + 1;
@@ -943,10 +982,13 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
   DDC_LAUNCHED();
 
   const int mn = std::min(m, n);
+  int fusedL = 0;  // tiles of left GEMV partials the trailing update left for step k, or 0
   for (int k = 0; k < mn; ++k) {
     a.k = k;
     a.p = (k / NB) * NB;
     a.t = k - a.p;
+    const int nt = fusedL;
+    fusedL = 0;
     if (m < n && k == m - 1) {
       const int nw = (int)ceil_div(n - k - 1, FIN);
       hipLaunchKernelGGL(bd_wide_row<TS>, dim3(nw), dim3(FIN), 0, st, a);
@@ -965,9 +1007,15 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     const int rb0 = (k / 64) * 64;
     const int ncb = (int)ceil_div(n - k - 1, CBL);
     const int ndl = (int)ceil_div(2 * a.t, DPW);
-    const int nchL = (int)ceil_div(m - rb0, RC);
+    int nchL = (int)ceil_div(m - rb0, RC);
     hipEvent_t ke0, ke1;
-    if (kclock_on() && kclock_pair(KCLOCK_LEFT_GEMV, &ke0, &ke1) == 0)
+    if (nt > 0) {
+      // the trailing update formed the partial sums (t = 0: no panel dots)
+      const int ncs = (int)ceil_div(n - k - 1, 256);
+      hipLaunchKernelGGL(bd_left_red<TS>, dim3(ncs + (unsigned)ceil_div(m - k, 256)), dim3(256),
+                         0, st, a, nt, ncs);
+      nchL = 1;
+    } else if (kclock_on() && kclock_pair(KCLOCK_LEFT_GEMV, &ke0, &ke1) == 0)
       hipExtLaunchKernelGGL((bd_left_gemv<TS, RC / 512>), dim3(ncb + ndl, nchL), dim3(256), 0,
                             st, ke0, ke1, 0, a, ncb, rb0);
     else
@@ -1009,8 +1057,14 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       const int r0 = a.p + NB;
       TS* C = A + r0 + (int64_t)r0 * lda;
       if constexpr (sizeof(TS) == 8) {
-        DDC_TRY(dgemm_mfma('N', 'T', m - r0, n - r0, 2 * NB, -1.0, UX + r0, lda, a.Y + r0,
-                           d.ldy, 1.0, C, lda, st));
+        // The update also takes the partial sums of the next step's left
+        // GEMV (column r0 of the updated matrix is c, from bd_right_fin)
+        // when that step is a plain one: one pass over the trailing matrix
+        // less per panel.
+        const bool fuse = r0 < n - 1 && !(m < n && r0 == m - 1);
+        DDC_TRY(dgemm_trail64(m - r0, n - r0, UX + r0, lda, a.Y + r0, d.ldy, C, lda,
+                              fuse ? a.cvec + r0 : nullptr, a.Lpart + r0, d.ldy, 1, st));
+        if (fuse) fusedL = (int)ceil_div(m - r0, TRAIL_TILE);
       } else {
         const float* Up = A + r0 + (int64_t)a.p * lda;
         const int ra = m - r0, rb = n - r0;

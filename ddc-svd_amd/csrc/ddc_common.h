@@ -187,6 +187,13 @@ int gemm_bf16_f32(char ta, char tb, int M, int N, int K, float alpha, const void
 // arguments as dgemm.
 int dgemm_mfma(char ta, char tb, int M, int N, int K, double alpha, const double* A, int lda,
                const double* B, int ldb, double beta, double* C, int ldc, hipStream_t st);
+// The bidiagonalization's trailing update C (M x N) -= A B^T with inner
+// dimension 64 (A: M x 64, ld lda; B: N x 64, ld ldb).  With cdot, also the
+// column sums of the updated C against cdot (length M) over each 128-row
+// tile I: part[I * ldp + j] for the columns j >= jfrom, in a fixed order.
+int dgemm_trail64(int M, int N, const double* A, int lda, const double* B, int ldb, double* C,
+                  int ldc, const double* cdot, double* part, int ldp, int jfrom, hipStream_t st);
+constexpr int TRAIL_TILE = 128;  // rows per partial of dgemm_trail64
 // B <- T^{-1} B for the M x M upper triangular T (non-unit diagonal).
 int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
                      hipStream_t st);

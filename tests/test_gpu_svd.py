@@ -69,8 +69,11 @@ def check_svd(A, s, U, V, sig_tol=1e-12, vec=True):
 
 # (2500, 200) and (4200, 130): several 2048-row chunks of the left GEMV and
 # several 512-row blocks of the right GEMV
+# (70, 66): the panel boundary k = 64 is column n-2, whose left GEMV partial
+# sums come from the trailing update and go to the last left finalization
 BIDIAG_SHAPES = [(1, 1), (5, 1), (2, 2), (3, 2), (3, 3), (8, 3), (33, 33), (64, 64),
-                 (96, 64), (100, 70), (130, 97), (300, 300), (2500, 200), (4200, 130)]
+                 (96, 64), (100, 70), (70, 66), (130, 97), (300, 300), (2500, 200),
+                 (4200, 130)]
 
 
 @pytest.mark.parametrize("shape", BIDIAG_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
@@ -88,7 +91,7 @@ def test_bidiag_matches_serial_reference(ddcsvd, po, shape):
 
 
 WIDE_SHAPES = [(1, 2), (1, 5), (2, 3), (3, 5), (5, 6), (33, 100), (64, 96), (200, 700),
-               (130, 300)]
+               (130, 300), (65, 90)]
 
 
 @pytest.mark.parametrize("shape", WIDE_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
