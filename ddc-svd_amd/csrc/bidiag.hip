@@ -392,7 +392,9 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
 }
 
 // right_gemv with the left finalization fused in.  grid.x = nrb row blocks
-// + 1 owner block, grid.y = column chunks of CC = 128 columns.
+// + 1 owner block, grid.y = column chunks of CC = 128 or 64 columns (the
+// narrower chunks for the smaller trailing matrices, where twice the
+// work-groups keep more loads in flight).
 //
 // Every work-group first forms row k of M on its chunk, as bd_left_fin does
 // for all columns (bitwise the same values in every work-group: same code,
@@ -405,12 +407,14 @@ __global__ __launch_bounds__(FIN) void bd_left_fin(StepArgs<TS> a, int nchL) {
 // v_s^T r (s < t) for right_fin.  This saves the launch of bd_left_fin.
 template <typename TS, int CC, int PF>
 __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, int rb0, int nchL) {
-  static_assert(CC == 128, "two threads per column of the chunk");
+  static_assert(CC == 128 || CC == 64, "a whole number of waves per part of the chunk");
   static_assert(PF % 8 == 0 && PF >= 8, "columns prefetched in batches of eight");
-  constexpr int H = NB / 2;             // panel columns per half
+  constexpr int TPC = 256 / CC;         // threads per column: the column's parts
+  constexpr int H = NB / TPC;           // panel columns per part
+  constexpr int WPP = CC / 64;          // waves per part
   __shared__ double rs[CC];             // r on the chunk
   __shared__ double yv[CC];             // y_t on the chunk
-  __shared__ double pq[2][CC];          // second half's P, Q
+  __shared__ double pq[TPC - 1][2][CC]; // the other parts' P, Q
   __shared__ double ds[2 * NB];         // u_s^T c, x_s^T c (s < t)
   __shared__ double uk[NB + 1];         // U[k, s], s <= t (u_t[k] from the reflector)
   __shared__ double xk[NB];             // X[k, s], s < t
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   }
   const int j0 = k + 1 + by * CC;
   const int jn = min(CC, a.n - j0);
-  const int col = tid & (CC - 1), half = tid >> 7;
+  const int col = tid & (CC - 1), part = tid / CC;
   const bool jv = col < jn;
   const int jc = jv ? j0 + col : k + 1;  // a valid column for the loads of idle lanes
 
@@ -452,12 +456,12 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
       for (int u = 0; u < 8; ++u) v0[i][u] = ld2(base + (int64_t)min(8 * i + u, jn - 1) * a.lda);
   }
 
-  // Every load first.  Thread (col, half) holds Y[j, s], V[j, s] for the
-  // panel columns s = half H + i < t.
+  // Every load first.  Thread (col, part) holds Y[j, s], V[j, s] for the
+  // panel columns s = part H + i < t.
   double ys[H] = {}, vs[H] = {};
-  const double* Yj = a.Y + jc + (int64_t)(half * H) * a.ldy;
-  const double* Vj = a.V + jc + (int64_t)(half * H) * a.ldy;
-  const int th = t - half * H;  // panel columns of this half
+  const double* Yj = a.Y + jc + (int64_t)(part * H) * a.ldy;
+  const double* Vj = a.V + jc + (int64_t)(part * H) * a.ldy;
+  const int th = t - part * H;  // panel columns of this part
 #pragma unroll
   for (int g = 0; g < H / 8; ++g) {
     if (g * 8 < th) {
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
     }
   }
   double gsum = 0.0, rk = 0.0;
-  if (half == 0) {
+  if (part == 0) {
     rk = (double)a.A[k + (int64_t)jc * a.lda];
     gsum = sum_strided<8>(a.Lpart + jc, nchL, a.ldy);  // A0[k:m, j]^T c
   }
@@ -498,29 +502,32 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   }
   __syncthreads();
 
-  // panel terms of this half
+  // panel terms of this part
   double P = 0.0, Q = 0.0;
 #pragma unroll
   for (int i = 0; i < H; ++i) {
     if (i < th) {
-      const int sx = half * H + i;
+      const int sx = part * H + i;
       P += ys[i] * ds[sx] + vs[i] * ds[t + sx];
       Q += ys[i] * uk[sx] + vs[i] * xk[sx];
     }
   }
-  if (half == 1) {
-    pq[0][col] = P;
-    pq[1][col] = Q;
+  if (part > 0) {
+    pq[part - 1][0][col] = P;
+    pq[part - 1][1][col] = Q;
   }
   __syncthreads();
   double y = 0.0, r = 0.0;
-  if (half == 0) {
+  if (part == 0) {
     Refl R;
     R.nrm = rfl[0];
     R.sgn = rfl[1];
     R.rl = rfl[2];
-    P += pq[0][col];
-    Q += pq[1][col];
+#pragma unroll
+    for (int q = 0; q < TPC - 1; ++q) {
+      P += pq[q][0][col];
+      Q += pq[q][1][col];
+    }
     y = 2.0 * R.dot(gsum - P, rk - Q);
     r = rk - Q - uk[t] * y;
     if (!jv) y = r = 0.0;
@@ -593,9 +600,9 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
     // owner: y_t and r, then the chunk's partial sums for right_fin in the
     // layout of Rdot: [s] y_s^T r (s <= t), [t + 1 + s] v_s^T r (s < t),
     // and |r|^2 in rnorm.  Slots of the reduction: 0..31 y_s^T r (slot t:
-    // y_t^T r), 32..62 v_s^T r, 63 |r|^2; this half holds slots
-    // half H + i and 32 + half H + i, i < H.
-    if (half == 0 && jv) {
+    // y_t^T r), 32..62 v_s^T r, 63 |r|^2; this part holds slots
+    // part H + i and 32 + part H + i, i < H.
+    if (part == 0 && jv) {
       a.Y[j0 + col + (int64_t)t * a.ldy] = y;
       a.rvec[j0 + col] = r;
     }
@@ -603,18 +610,21 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
     double g[2 * H];
 #pragma unroll
     for (int i = 0; i < H; ++i) {
-      const int sx = half * H + i;
+      const int sx = part * H + i;
       g[i] = (i < th) ? ys[i] * rr : (sx == t ? yy * rr : 0.0);
       g[H + i] = (i < th) ? vs[i] * rr : (sx == NB - 1 ? rr * rr : 0.0);
     }
-    const double tot = reduce_scatter<double, 2 * H>(g);  // lane l: slot index l >> 1
-    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
+    constexpr int LPS = 64 / (2 * H);  // lanes per slot after the reduce-scatter
+    const double tot = reduce_scatter<double, 2 * H>(g);  // lane l: slot index l / LPS
+    if ((lane & (LPS - 1)) == 0) red[wave][lane / LPS] = tot;
     __syncthreads();
     if (tid < 64) {
-      const int hf = tid >> 5, i = tid & 31;  // half, index within the half's 2H slots
-      const double v = red[2 * hf][i] + red[2 * hf + 1][i];
+      const int pt = tid / (2 * H), i = tid % (2 * H);  // part, index within its 2H slots
+      double v = red[pt * WPP][i];
+#pragma unroll
+      for (int w = 1; w < WPP; ++w) v += red[pt * WPP + w][i];
       const bool ysl = i < H;
-      const int sx = hf * H + (ysl ? i : i - H);
+      const int sx = pt * H + (ysl ? i : i - H);
       double* Rd = a.Rdot + (int64_t)by * DSTRIDE;
       if (ysl) {
         if (sx <= t) Rd[sx] = v;
@@ -840,7 +850,7 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
   // left GEMV partials: one per 2048-row chunk, or one per 128-row tile of
   // the trailing update that carries the next column's left GEMV
   d.nchL_max = (int)std::max(ceil_div(lda, RC), ceil_div(lda, TRAIL_TILE)) + 1;
-  d.nccR_max = (int)ceil_div(n, CCR) + 1;
+  d.nccR_max = (int)ceil_div(n, 64) + 1;  // 64- or 128-column chunks
   d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(std::max(m, n), FIN)) + 1;
   d.nrn_max = (int)ceil_div(n, FIN) + 1;
   return d;
@@ -925,6 +935,34 @@ void bidiag_plan_f32(ArenaPlan& p, int m, int n, int lda, int upd) {
     p.add<unsigned short>((size_t)lda * 2 * NB * pieces);
     p.add<unsigned short>((size_t)ldy * 2 * NB * pieces);
   }
+}
+
+// Trailing widths (columns right of k) up to which the right GEMV takes
+// 64-column chunks: there 128-column chunks leave too few work-groups to
+// keep HBM busy.  Per 8192^2 SVD, by threshold: 0 (never) 654.3 ms, 1024
+// 652.2, 2048 650.2, 3072 650.8, 4096 649.7, 8192 (always) 656.9, where
+// the right GEMV itself is fastest (30.8 against 32.3 us per launch) but
+// bd_right_fin sums 128 chunk partials per row at the largest widths.
+constexpr int RIGHT_NARROW_COLS = 4096;
+
+// The right GEMV of step a.k; returns its chunk count.  16 columns are
+// prefetched before the prologue (8: 1.5-2 ms more per 8192^2 SVD; 24
+// would lower the occupancy to two waves per SIMD).
+template <typename TS>
+static int launch_right_gemv(const StepArgs<TS>& a, int nchL, bool clock, hipStream_t st) {
+  const int k = a.k;
+  const int rbr = ((k + 1) / 64) * 64;
+  const int nrb = (int)ceil_div(a.m - rbr, RBR);
+  const bool narrow = a.n - k - 1 <= RIGHT_NARROW_COLS;
+  const int ncc = (int)ceil_div(a.n - k - 1, narrow ? 64 : CCR);
+  auto kern = narrow ? bd_right_gemv<TS, 64, 16> : bd_right_gemv<TS, CCR, 16>;
+  hipEvent_t ke0, ke1;
+  if (clock && kclock_on() && kclock_pair(KCLOCK_RIGHT_GEMV, &ke0, &ke1) == 0)
+    hipExtLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, ke0, ke1, 0, a, nrb, rbr,
+                          nchL);
+  else
+    hipLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, a, nrb, rbr, nchL);
+  return ncc;
 }
 
 template <typename TS>
@@ -1035,17 +1073,7 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
       DDC_LAUNCHED();
     } else {
-      const int rbr = ((k + 1) / 64) * 64;
-      const int ncc = (int)ceil_div(n - k - 1, CCR);
-      const int nrb = (int)ceil_div(m - rbr, RBR);
-      // 16 columns prefetched before the prologue (8: 1.5-2 ms more per
-      // 8192^2 SVD; 24 would lower the occupancy to two waves per SIMD)
-      auto kern = bd_right_gemv<TS, CCR, 16>;
-      if (kclock_on() && kclock_pair(KCLOCK_RIGHT_GEMV, &ke0, &ke1) == 0)
-        hipExtLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, ke0, ke1, 0, a, nrb,
-                              rbr, nchL);
-      else
-        hipLaunchKernelGGL(kern, dim3(nrb + 1, ncc), dim3(256), 0, st, a, nrb, rbr, nchL);
+      const int ncc = launch_right_gemv(a, nchL, true, st);
       DDC_LAUNCHED();
       a.nrn = ncc;
       hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
@@ -1204,11 +1232,7 @@ int bidiag_right_gemv_only(int m, int n, double* A, int lda, double* alpha, int 
     // is at most that of step 0
     const int rb0 = (k / 64) * 64;
     const int nchL = (int)ceil_div(m - rb0, RC);
-    const int rbr = ((k + 1) / 64) * 64;
-    const int ncc = (int)ceil_div(n - k - 1, CCR);
-    const int nrb = (int)ceil_div(m - rbr, RBR);
-    hipLaunchKernelGGL((bd_right_gemv<double, CCR, 16>), dim3(nrb + 1, ncc), dim3(256), 0, st,
-                       a, nrb, rbr, nchL);
+    launch_right_gemv(a, nchL, false, st);
     DDC_LAUNCHED();
   }
   return DDC_OK;
