@@ -684,10 +684,11 @@ def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n:
 def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
     """Reads the counter CSV of a --pmc-pass gemm run (MfmaUtil and
     SQ_INSTS_VALU_MFMA_MOPS_F64 with the kernel trace) and writes
-    profiles/k_dgemm_{m}x{n}_mfma.md: per dispatch of the hand-written FP64
-    MFMA GEMM, MFMA FLOPs = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (the counter
-    definitions' MfmaFlopsF64), duration from the dispatch timestamps,
-    MfmaUtil = MFMA-busy cycles / (GPU-busy cycles x SIMDs)."""
+    profiles/k_trail64_{m}x{n}_mfma.md: per dispatch of the hand-written
+    FP64 MFMA trailing update (k_trail64) and of the general GEMM (k_dgemm,
+    the 8192^3 product), MFMA FLOPs = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (the
+    counter definitions' MfmaFlopsF64), duration from the dispatch
+    timestamps, MfmaUtil = MFMA-busy cycles / (GPU-busy cycles x SIMDs)."""
     import csv
     import glob
 
@@ -699,11 +700,13 @@ def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
             d[r["Counter_Name"]] = float(r["Counter_Value"])
             d["name"] = r["Kernel_Name"]
             d["ns"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
-    ks = [v for _, v in sorted(per.items()) if "k_dgemm" in v["name"]]
-    if len(ks) < 2:
-        print("no k_dgemm dispatches found", file=sys.stderr)
+    rows = [v for _, v in sorted(per.items())]
+    upd = [v for v in rows if "k_trail64" in v["name"]]
+    bigs = [v for v in rows if "k_dgemm" in v["name"]]
+    if not upd or not bigs:
+        print("no k_trail64 / k_dgemm dispatches found", file=sys.stderr)
         return 1
-    upd, big = ks[:-1], ks[-1]
+    big = bigs[-1]
     peak = 78.6  # TFLOP/s, MI355X dense FP64 MFMA
 
     def line(label, v, alg_flops, alg_bytes):
@@ -714,7 +717,7 @@ def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
                 f"{alg_bytes / us / 1e3:.0f} |")
 
     shapes = [(m - r0, n - r0) for r0 in range(32, n, 32)]
-    out = [f"# FP64 MFMA GEMM (k_dgemm): trailing updates of a {m}x{n} bidiagonalization",
+    out = [f"# FP64 MFMA trailing update (k_trail64) of a {m}x{n} bidiagonalization",
            "",
            "Command (MI355X, ROCm 7.2, one GPU; run from the repository root with "
            "`TMPDIR=/tmp`):",
@@ -725,8 +728,9 @@ def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
            "",
            "The workload: the trailing updates C (m-r0 x n-r0, leading dimension of the SVD's "
            "working copy) -= [U X] [Y V]^T with inner dimension 64 for r0 = 32, 64, ... "
-           "(svd_gpu_dgemm, the kernel the bidiagonalization calls), then one 8192^3 GEMM "
-           "for scale.  MFMA GFLOP = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (whole 128 x 128 "
+           "(svd_gpu_dgemm, which takes the bidiagonalization's kernel k_trail64 for this "
+           "shape; the SVD's own calls also form the next left GEMV's partial sums), then one "
+           "8192^3 GEMM (k_dgemm) for scale.  MFMA GFLOP = 512 x SQ_INSTS_VALU_MFMA_MOPS_F64 (whole 128 x 128 "
            "tiles, so above the algorithmic 2 m n k at ragged edges); % of peak against "
            f"{peak} TFLOP/s; MfmaUtil from the counter definitions (MFMA-busy cycles over "
            "GPU-busy cycles x SIMDs); GB/s = the update's algorithmic bytes (C read and "
@@ -751,13 +755,14 @@ def summarize_mfma(trace_dir: str, m: int, n: int) -> int:
             f"GFLOP ({tot_fl / 1e9:.1f} on the MFMA units) -> {alg_fl / tot_ns / 1e3:.1f} TFLOP/s "
             f"({100 * alg_fl / tot_ns / 1e3 / peak:.1f}% of FP64 peak), time-weighted MfmaUtil "
             f"{util:.1f}%, {alg_b / tot_ns:.0f} GB/s algorithmic ({100 * alg_b / tot_ns / 8000:.0f}% "
-            "of 8 TB/s): the update (inner dimension 64) is bound by the trailing matrix's read "
-            "and write, not by the matrix cores.", ""]
+            "of 8 TB/s).  At inner dimension 64 a 128 x 128 tile needs about as long on the "
+            "FP64 matrix cores as its C traffic needs at a CU's share of HBM; the kernel "
+            "overlaps the two (the next tile's loads during this tile's products).", ""]
     out.append("| dispatch | us | alg. GFLOP | MFMA GFLOP | TFLOP/s | % of FP64 peak | MfmaUtil % "
                "| alg. GB/s |")
     out.append("|---|---|---|---|---|---|---|---|")
-    out.append(line("8192^3 GEMM", big, 2.0 * 8192 ** 3, 8.0 * 3 * 8192 ** 2))
-    path = os.path.join(ROOT, "profiles", f"k_dgemm_{m}x{n}_mfma.md")
+    out.append(line("8192^3 GEMM (k_dgemm)", big, 2.0 * 8192 ** 3, 8.0 * 3 * 8192 ** 2))
+    path = os.path.join(ROOT, "profiles", f"k_trail64_{m}x{n}_mfma.md")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
     print("\n".join(out))

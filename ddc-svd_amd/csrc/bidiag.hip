@@ -637,18 +637,30 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   }
 }
 
-// right_fin: one thread per row in [k+1, m), one wave per group; thread i
-// of the grid also stores entry k+1+i of the right reflector (m >= n, so
-// the grid covers the row).  noright: the row of M has a single entry,
-// taken as beta with no reflection (bidiag.c:125-129).
+// right_fin: one row in [k+1, m) per lane, FIN rows per group of four
+// waves; lane i of the grid also stores entry k+1+i of the right reflector
+// (the grid covers max(m, n) - k - 1 rows).  The four waves share the
+// loads of a row: wave w takes the panel columns s = 8w .. 8w+7 and the
+// chunk partials q = w, w + 4, ..., so that every load of the step is
+// issued at once with 16 + 32 loads per lane (one wave per group took 100
+// and a second round trip above 32 chunks); the partial sums meet in LDS.
+// noright: the row of M has a single entry, taken as beta with no
+// reflection (bidiag.c:125-129).
 template <typename TS>
-__global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright) {
+__global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright) {
+  constexpr int PW = NB / 4;  // panel columns per wave
+  constexpr int PB = 16;      // chunk partials per wave and batch
   __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
   __shared__ double yk1[NB];       // Y[k+1, s], s <= t
-  __shared__ double vk1[NB];       // V[k+1, s], s <= t
-  const int tid = threadIdx.x;
+  __shared__ double vk1[NB];       // V[k+1, s], s < t; [t]: v[k+1]
+  __shared__ double hp[4][FIN];    // per wave: partial A0[row, k+1:n] r
+  __shared__ double dp[4][FIN];    // per wave: partial panel dots with r
+  __shared__ double xp[4][FIN];    // per wave: panel terms of x
+  __shared__ double cp[4][FIN];    // per wave: panel terms of c
+  __shared__ double rfl[3];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k, t = a.t;
-  const int row = k + 1 + blockIdx.x * FIN + tid;
+  const int row = k + 1 + blockIdx.x * FIN + lane;
   const bool rv = row < a.m;
   const int rc = rv ? row : k + 1;  // a valid row for the loads of idle lanes
   const bool next = k + 1 < a.n;
@@ -656,106 +668,122 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
   // Every load first (see bd_left_fin).
   const TS* Ur = a.A + rc + (int64_t)a.p * a.lda;
   const double* Xr = a.X + rc;
-  double us[NB], xs[NB];
+  const int s0 = PW * wave;
+  double us[PW], xs[PW];
+  if (s0 < t) {
 #pragma unroll
-  for (int g = 0; g < NB / 8; ++g) {
-    if (g * 8 < t) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int s = min(g * 8 + u, t - 1);  // clamped: in bounds, unused past t
-        us[g * 8 + u] = (double)Ur[(int64_t)s * a.lda];
-        xs[g * 8 + u] = Xr[(int64_t)s * a.lda];
-      }
+    for (int i = 0; i < PW; ++i) {
+      const int s = min(s0 + i, t - 1);  // clamped: in bounds, unused past t
+      us[i] = (double)Ur[(int64_t)s * a.lda];
+      xs[i] = Xr[(int64_t)s * a.lda];
     }
   }
-  const double ut = (double)Ur[(int64_t)t * a.lda];
-  double c = next ? (double)a.A[rc + (int64_t)(k + 1) * a.lda] : 0.0;
-  const int nrn = noright ? 0 : a.nrn;
-  const double rn0 = a.rnorm[min(tid, max(nrn - 1, 0))];  // used if tid < nrn
-  const double rk1 = a.rvec[k + 1];
-  // this thread's column of the right reflector (written below)
-  const int jw = k + 1 + blockIdx.x * FIN + tid;
-  const double rw = a.rvec[jw < a.n ? jw : k + 1];
-  const bool dotr = !noright && tid < 2 * t + 1;
-  const int td = dotr ? tid : 0;
-  const double w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy]
-                              : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
-  // row k+1 of the panel's Y and V (to LDS below), with the batch too
-  const int tk = min(tid, t);
-  const double yk1v = a.Y[(k + 1) + (int64_t)tk * a.ldy];
-  const double vk1v = a.V[(k + 1) + (int64_t)tk * a.ldy];
-  // the first PB chunk partials of the panel dots (Rdot) and of this row's
-  // A0 r (Rpart), clamped to the last chunk, in the same batch as the loads
-  // above: one wait for all of them
-  constexpr int PB = 32;
+  const bool dotr = !noright && lane < 2 * t + 1;
+  const int td = dotr ? lane : 0;
   const int ncl = noright ? 1 : nccR;
   double pd[PB], ph[PB];
 #pragma unroll
   for (int u = 0; u < PB; ++u) {
-    const int q = min(u, ncl - 1);
+    const int q = min(wave + 4 * u, ncl - 1);
     pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
     ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
   }
+  const int nrn = noright ? 0 : a.nrn;
+  double c = 0.0, ut = 0.0, rk1 = 0.0, rw = 0.0, w1 = 0.0, rn = 0.0, yk = 0.0, vk = 0.0;
+  const int jw = k + 1 + blockIdx.x * FIN + lane;  // this lane's column of the reflector
+  if (wave == 0) {
+    c = next ? (double)a.A[rc + (int64_t)(k + 1) * a.lda] : 0.0;
+    ut = (double)Ur[(int64_t)t * a.lda];
+    rk1 = a.rvec[k + 1];
+    rw = a.rvec[jw < a.n ? jw : k + 1];
+    w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy] : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
+    rn = a.rnorm[min(lane, max(nrn - 1, 0))];  // used if lane < nrn
+  } else if (wave == 1) {
+    const int tk = min(lane, t);
+    yk = a.Y[(k + 1) + (int64_t)tk * a.ldy];
+    vk = a.V[(k + 1) + (int64_t)tk * a.ldy];
+  }
   __builtin_amdgcn_sched_barrier(0);
-  // sums in chunk order (as a sequential sum over the chunks)
-  double rn = 0.0, ds = 0.0, h = 0.0;  // h: A0[row, k+1:n] r
+
+  // partial sums of this wave's chunks, in chunk order
+  double h = 0.0, ds = 0.0;
   if (!noright) {
-    if (tid < nrn) rn += rn0;
-    for (int i = tid + FIN; i < nrn; i += FIN) rn += a.rnorm[i];
 #pragma unroll
     for (int u = 0; u < PB; ++u)
-      if (u < nccR) {
+      if (wave + 4 * u < nccR) {
         ds += pd[u];
         h += ph[u];
       }
-    for (int q = PB; q < nccR; q += PB) {
+    for (int q0 = 4 * PB; q0 < nccR; q0 += 4 * PB) {
 #pragma unroll
       for (int u = 0; u < PB; ++u) {
-        const int qq = min(q + u, nccR - 1);
-        pd[u] = a.Rdot[td + (int64_t)qq * DSTRIDE];
-        ph[u] = a.Rpart[rc + (int64_t)qq * a.lda];
+        const int q = min(q0 + wave + 4 * u, nccR - 1);
+        pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
+        ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
       }
 #pragma unroll
       for (int u = 0; u < PB; ++u)
-        if (q + u < nccR) {
+        if (q0 + wave + 4 * u < nccR) {
           ds += pd[u];
           h += ph[u];
         }
     }
   }
-
+  hp[wave][lane] = h;
+  dp[wave][lane] = ds;
   Refl R;
   R.nrm = R.sgn = R.rl = 0.0;
-  if (!noright) R.make(wave_sum(rn), rk1);
-  if (dotr) dyv[tid] = R.dot(ds, w1);
-  double x = R.dot(h, c);  // A0[row, k+1:n] v, with c = A0[row, k+1]; 0 if noright
-  if (tid <= t) yk1[tid] = yk1v;
-  if (tid < t) vk1[tid] = vk1v;
-  if (tid == t) vk1[t] = noright ? 0.0 : R.vec(rk1, true);
+  if (wave == 0) {
+    double r2 = lane < nrn ? rn : 0.0;
+    for (int i = lane + FIN; i < nrn; i += FIN) r2 += a.rnorm[i];
+    if (!noright) R.make(wave_sum(r2), rk1);
+    if (lane == 0) {
+      rfl[0] = R.nrm;
+      rfl[1] = R.sgn;
+      rfl[2] = R.rl;
+      vk1[t] = noright ? 0.0 : R.vec(rk1, true);
+    }
+  } else if (wave == 1) {
+    if (lane <= t) yk1[lane] = yk;
+    if (lane < t) vk1[lane] = vk;
+  }
   __syncthreads();
+  if (wave == 0 && dotr) dyv[lane] = R.dot(dp[0][lane] + dp[1][lane] + dp[2][lane] + dp[3][lane], w1);
+  __syncthreads();
+
+  // this wave's panel terms of x and c
+  double xc = 0.0, cc = 0.0;
+#pragma unroll
+  for (int i = 0; i < PW; ++i) {
+    const int s = s0 + i;
+    if (s < t) {
+      if (!noright) xc += us[i] * dyv[s] + xs[i] * dyv[t + 1 + s];
+      cc += us[i] * yk1[s] + xs[i] * vk1[s];
+    }
+  }
+  xp[wave][lane] = xc;
+  cp[wave][lane] = cc;
+  __syncthreads();
+  if (wave != 0) return;
 
   double cc2 = 0.0;
   if (rv) {
-#pragma unroll
-    for (int s = 0; s < NB; ++s) {
-      if (s < t) {
-        if (!noright) x -= us[s] * dyv[s] + xs[s] * dyv[t + 1 + s];
-        c -= us[s] * yk1[s] + xs[s] * vk1[s];
-      }
-    }
+    double x = R.dot(hp[0][lane] + hp[1][lane] + hp[2][lane] + hp[3][lane], c);  // 0 if noright
+    x -= xp[0][lane] + xp[1][lane] + xp[2][lane] + xp[3][lane];
     if (!noright) {
       x -= ut * dyv[t];
       x *= 2.0;
     }
     a.X[row + (int64_t)t * a.lda] = x;
     if (next) {
+      c -= cp[0][lane] + cp[1][lane] + cp[2][lane] + cp[3][lane];
       c -= ut * yk1[t] + x * vk1[t];
       a.cvec[row] = c;
       cc2 = c * c;
     }
   }
-  double s = wave_sum(cc2);
-  if (tid == 0) a.cnorm[blockIdx.x] = s;
+  const double s = wave_sum(cc2);
+  if (lane == 0) a.cnorm[blockIdx.x] = s;
   // the right reflector to row k of A and to column t of V (no work-group
   // of this launch reads either: rows > k of A, V[k+1, s < t])
   if (!noright) {
@@ -764,9 +792,9 @@ __global__ __launch_bounds__(FIN) void bd_right_fin(StepArgs<TS> a, int nccR, in
       a.A[k + (int64_t)jw * a.lda] = (TS)v;
       a.V[jw + (int64_t)t * a.ldy] = v;
     }
-    if (blockIdx.x == 0 && tid == 0) a.beta[k] = R.diag();
+    if (blockIdx.x == 0 && lane == 0) a.beta[k] = R.diag();
   }
-  if (noright && blockIdx.x == 0 && tid == 0) {
+  if (noright && blockIdx.x == 0 && lane == 0) {
     a.beta[k] = a.rvec[k + 1];
     a.A[k + (int64_t)(k + 1) * a.lda] = (TS)0;
     a.V[(k + 1) + (int64_t)t * a.ldy] = 0.0;
@@ -1070,13 +1098,13 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);
       DDC_LAUNCHED();
       a.nrn = nfl;
-      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, 0, 1);
+      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(4 * FIN), 0, st, a, 0, 1);
       DDC_LAUNCHED();
     } else {
       const int ncc = launch_right_gemv(a, nchL, true, st);
       DDC_LAUNCHED();
       a.nrn = ncc;
-      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(FIN), 0, st, a, ncc, 0);
+      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(4 * FIN), 0, st, a, ncc, 0);
       DDC_LAUNCHED();
     }
     a.ncn = nfr;
