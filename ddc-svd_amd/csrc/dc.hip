@@ -188,7 +188,11 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
   double* wL = B.wL + off;
   int* flag = B.flag + off;
 
-  // 1. rank-merge the two ascending child spectra behind the pole at 0
+  // 1. merge the two ascending child spectra behind the pole at 0 (on ties
+  // the first child's value first).  Merge path: thread t writes the merged
+  // outputs [t per, (t+1) per), finding its start in the two lists by one
+  // binary search and then walking both (a binary search per element was a
+  // chain of 13 dependent loads per element, most of this kernel's time).
   double zmax = 0.0;
   if (tid == 0) {
     wd[0] = 0.0;
@@ -197,21 +201,37 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
     wL[0] = sg * ps2;
     zmax = fabs(r0);
   }
-  for (int i = tid; i < K; i += 256) {
-    int pos = 1 + i + count_less(s2, N2, s1[i]);
-    wd[pos] = s1[i];
-    wz[pos] = aK * l1[i];
-    wF[pos] = f1[i];
-    wL[pos] = 0.0;
-    zmax = fmax(zmax, fabs(aK * l1[i]));
-  }
-  for (int j = tid; j < N2; j += 256) {
-    int pos = 1 + j + count_leq(s1, K, s2[j]);
-    wd[pos] = s2[j];
-    wz[pos] = bK * f2[j];
-    wF[pos] = 0.0;
-    wL[pos] = l2[j];
-    zmax = fmax(zmax, fabs(bK * f2[j]));
+  {
+    const int NO = K + N2;  // merged outputs after the pole
+    const int per = (NO + 255) / 256;
+    const int d0 = min(NO, tid * per), d1 = min(NO, d0 + per);
+    int lo = max(0, d0 - N2), hi = min(d0, K);
+    while (lo < hi) {  // i = the number of first-child values among outputs < d0
+      const int mid = (lo + hi) >> 1;
+      if (s1[mid] <= s2[d0 - mid - 1]) lo = mid + 1;
+      else hi = mid;
+    }
+    int i = lo, j = d0 - lo;
+    for (int p = d0; p < d1; ++p) {
+      const bool first = j >= N2 || (i < K && s1[i] <= s2[j]);
+      if (first) {
+        const double zv = aK * l1[i];
+        wd[1 + p] = s1[i];
+        wz[1 + p] = zv;
+        wF[1 + p] = f1[i];
+        wL[1 + p] = 0.0;
+        zmax = fmax(zmax, fabs(zv));
+        ++i;
+      } else {
+        const double zv = bK * f2[j];
+        wd[1 + p] = s2[j];
+        wz[1 + p] = zv;
+        wF[1 + p] = 0.0;
+        wL[1 + p] = l2[j];
+        zmax = fmax(zmax, fabs(zv));
+        ++j;
+      }
+    }
   }
   zmax = group_max(zmax, red);
   __syncthreads();
@@ -229,8 +249,14 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
   __syncthreads();
 
   // 3. close poles: sequential rotations along the sorted kept entries.
-  // Chains of close poles are rare; one thread walks the list.
-  if (tid == 0) {
+  // Chains of close poles are rare; one thread walks the list, and only if
+  // some kept entry lies within tol of its predecessor in the sorted list
+  // (else no pair of kept entries is that close and the walk rotates
+  // nothing).
+  int close = 0;
+  for (int j = 1 + tid; j < N; j += 256) close |= (!flag[j] && wd[j] - wd[j - 1] <= tol) ? 1 : 0;
+  close = __syncthreads_or(close);
+  if (tid == 0 && close) {
     int prev = 0;
     for (int j = 1; j < N; ++j) {
       if (flag[j]) continue;
@@ -263,9 +289,9 @@ __global__ __launch_bounds__(256) void dc_prep(DcBufs B, int mbase) {
       }
       prev = j;
     }
-    // a zero arrow (every pole and z at zero) keeps its root at zero
-    if (fabs(wz[0]) <= tol) wz[0] = s_scale > 0.0 ? tol : 0.0;
   }
+  // a zero arrow (every pole and z at zero) keeps its root at zero
+  if (tid == 0 && fabs(wz[0]) <= tol) wz[0] = s_scale > 0.0 ? tol : 0.0;
   __syncthreads();
 
   // 4. compaction: kept entries to [0, K'), deflated to [0, N-K') in order
