@@ -12,15 +12,16 @@ point svd_gpu_dev, input already resident in HBM.  Default: 8192 x 8192.
 
 Prints one JSON line.  `roofline` covers the dominant kernel of the SVD, the
 one with the largest total time per SVD among the bidiagonalization's
-GEMVs: the right GEMV (bd_right_gemv), whose algorithmic traffic is one FP64
-read of the trailing block A[k+1:m, k+1:n] per launch, 8 (m-k-1)(n-k-1)
-bytes; achieved = the sum of those bytes / the sum of the launch durations
-over the timed steps.  The durations are stamped by the hardware at the
-start and the end of each launch (hipExtLaunchKernel with start and stop
-events, on the library's stream; svd_gpu_profile_kernels), in extra steps
-kept out of `value`.  The left GEMV (bd_left_gemv, 8 (m-k)(n-k-1) bytes per
-launch) is reported beside it in `kernels`, with the counter-measured HBM
-traffic of profiles/bd_left_gemv_{m}x{n}_traffic.json.
+GEMVs: the one-pass GEMV (bd_fused_gemv: both products of a column step in
+one read of the trailing block A[k:m, k+1:n], 8 (m-k)(n-k-1) bytes per
+launch); achieved = the sum of those bytes / the sum of the launch
+durations over the timed steps.  The durations are stamped by the hardware
+at the start and the end of each launch (hipExtLaunchKernel with start and
+stop events, on the library's stream; svd_gpu_profile_kernels), in extra
+steps kept out of `value`.  The two-pass GEMVs (bd_left_gemv, bd_right_gemv)
+of the steps that do not run as one pass are reported beside it in
+`kernels`; `traffic` is the counter-measured HBM traffic of
+profiles/<kernel>_{m}x{n}_traffic.json.
 `cpu_baseline` is the reference's host path (the oracle restatement,
 bit-identical to the compiled reference) on the box's host cores: the
 serial bidiagonalization (one core by construction) timed on the first
@@ -174,19 +175,46 @@ def cpu_lapack(m: int, n: int, seed: int) -> dict:
     }
 
 
-def left_gemv_bytes(m: int, n: int, fused: bool = True) -> int:
-    """Algorithmic bytes of all left GEMV launches: sum_{k<n-1} 8 (m-k)(n-k-1)
-    over the steps that launch one (m >= n).  fused: in the SVD the first
-    column of every panel after the first (k = 32, 64, ...) takes its partial
-    sums from the trailing update instead (the profiling hook launches every
-    step)."""
-    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 1)
-               if not fused or k == 0 or k % 32 != 0)
+FG_ROWS = 8192  # rows up to which a column step runs as one pass (bidiag.hip)
+
+
+def one_pass(m: int, n: int, k: int) -> bool:
+    """Whether column step k (m >= n) runs its two GEMVs as one pass."""
+    return k < n - 2 and m - k <= FG_ROWS and n - k - 1 <= 256 * 512
+
+
+def gemv_bytes(m: int, n: int) -> dict:
+    """Algorithmic bytes per kernel over the column steps of one m x n
+    (m >= n) bidiagonalization, as the driver chooses them: a one-pass step
+    reads its trailing block A0[k:m, k+1:n] once, 8 (m-k)(n-k-1) bytes; a
+    two-pass step has a left GEMV of 8 (m-k)(n-k-1) bytes (none at a panel's
+    first column after the first, whose partial sums the trailing update
+    forms) and, below k = n-2, a right GEMV of 8 (m-k-1)(n-k-1)."""
+    out = {"bd_fused_gemv": 0, "bd_left_gemv": 0, "bd_right_gemv": 0}
+    for k in range(n - 1):
+        if one_pass(m, n, k):
+            out["bd_fused_gemv"] += 8 * (m - k) * (n - k - 1)
+            continue
+        if not (k > 0 and k % 32 == 0):
+            out["bd_left_gemv"] += 8 * (m - k) * (n - k - 1)
+        if k < n - 2:
+            out["bd_right_gemv"] += 8 * (m - k - 1) * (n - k - 1)
+    return out
+
+
+def left_gemv_bytes(m: int, n: int) -> int:
+    """Algorithmic bytes of the left-GEMV hook's launches, one per step k < n-1."""
+    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 1))
 
 
 def right_gemv_bytes(m: int, n: int) -> int:
-    """Algorithmic bytes of all right GEMV launches: sum_{k<n-2} 8 (m-k-1)(n-k-1)."""
+    """Algorithmic bytes of the right-GEMV hook's launches: sum_{k<n-2} 8 (m-k-1)(n-k-1)."""
     return sum(8 * (m - k - 1) * (n - k - 1) for k in range(n - 2))
+
+
+def fused_gemv_bytes(m: int, n: int) -> int:
+    """Algorithmic bytes of the one-pass hook's launches: sum_{k<n-2} 8 (m-k)(n-k-1)."""
+    return sum(8 * (m - k) * (n - k - 1) for k in range(n - 2))
 
 
 def traffic_summary(m: int, n: int, kernel: str = "bd_left_gemv"):
@@ -230,7 +258,8 @@ def plain_form(m: int, n: int, seed: int = 2, steps: int = 3) -> dict:
     return {"config": f"plain FP64 svd_gpu {m}x{n}, N(0,1) entries, seed {seed}, "
                       f"device-resident input", "ms": 1e3 * sec, "svd_per_s": 1.0 / sec,
             "steps": steps, "stages_ms": {k: stage[k] for k in ddcsvd.TIMING_KEYS[:6]},
-            "bidiag_GBs": 16.0 * S(m, n) / (stage["bidiag"] / 1e3) / 1e9,
+            "bidiag_GBs": sum(gemv_bytes(max(m, n), min(m, n)).values())
+                          / (stage["bidiag"] / 1e3) / 1e9,
             "residual": res, "orth_u": ou, "orth_v": ov}
 
 
@@ -438,6 +467,9 @@ def pmc_pass(kernel: str, m: int, n: int, seed: int) -> int:
     if kernel == "left":
         print(f"pmc pass: {n - 1} left-GEMV launches, {m}x{n}", flush=True)
         ddcsvd.profile_left_gemv(A, n - 1, stream)
+    elif kernel == "fused":
+        print(f"pmc pass: {n - 2} one-pass GEMV launches, {m}x{n}", flush=True)
+        ddcsvd.profile_fused_gemv(A, n - 2, stream)
     else:
         print(f"pmc pass: {n - 2} right-GEMV launches, {m}x{n}", flush=True)
         ddcsvd.profile_right_gemv(A, n - 2, stream)
@@ -471,8 +503,8 @@ def summarize_pmc(kernel: str, fetch_dir: str, write_dir: str, m: int, n: int) -
         return per, names
 
     launches = n - 1 if kernel == "left" else n - 2
-    alg = (left_gemv_bytes(m, n, fused=False) if kernel == "left"
-           else right_gemv_bytes(m, n)) / launches
+    alg = {"left": left_gemv_bytes, "right": right_gemv_bytes,
+           "fused": fused_gemv_bytes}[kernel](m, n) / launches
     out = {"workload": f"the {launches} {kernel}-GEMV launches of one {m}x{n} N(0,1) "
                        f"bidiagonalization (svd_gpu_profile_{kernel}_gemv), seed 2",
            "commands": [f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run --output-format csv "
@@ -614,9 +646,9 @@ def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n:
         a[1] += (t1 - t0) / 1e3
         if "bd_init" in nm:
             k = -1
-        if "bd_left_gemv" in nm or "bd_left_red" in nm:
+        if "bd_left_gemv" in nm or "bd_left_red" in nm or "bd_fused_gemv" in nm:
             k += 1
-        for kern in ("bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
+        for kern in ("bd_fused_gemv", "bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
             if kern in nm and k >= 0:
                 b = by_range.setdefault((kern, k // 1024), [0, 0.0])
                 b[0] += 1
@@ -643,10 +675,11 @@ def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n:
                    f"{100 * us / tot_us:5.1f}")
     out.append(f"total kernel time per SVD: {tot_us / nsvd / 1e3:.1f} ms")
     out += ["", "bidiagonalization kernels by column step (average us per launch)",
-            f"{'steps':>13s} {'bd_left_gemv':>14s} {'bd_right_gemv':>14s} {'bd_right_fin':>14s}"]
+            f"{'steps':>13s} {'bd_fused_gemv':>14s} {'bd_left_gemv':>14s} {'bd_right_gemv':>14s} "
+            f"{'bd_right_fin':>14s}"]
     for rng in range((steps + 1023) // 1024):
         cells = []
-        for kern in ("bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
+        for kern in ("bd_fused_gemv", "bd_left_gemv", "bd_right_gemv", "bd_right_fin"):
             c, us = by_range.get((kern, rng), (0, 0.0))
             cells.append(f"{us / c:14.2f}" if c else f"{'-':>14s}")
         lo, hi = 1024 * rng, min(1024 * rng + 1023, steps - 1)
@@ -889,7 +922,7 @@ def mixed_gaussian(n: int = 8192, seed: int = 2) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pmc-pass", choices=["left", "right", "gemm", "backtrans"],
+    ap.add_argument("--pmc-pass", choices=["left", "right", "fused", "gemm", "backtrans"],
                     help="workload of one rocprofv3 --pmc pass (copy calibration + one "
                          "GEMV kernel's launches, or the trailing-update GEMMs)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("KERNEL", "FETCH_DIR", "WRITE_DIR"),
@@ -979,8 +1012,12 @@ def main() -> int:
 
     mt, nt = max(m, n), mn
     gemv = {}
-    for name, nbytes, key in (("bd_left_gemv", left_gemv_bytes(mt, nt), "left_gemv"),
-                              ("bd_right_gemv", right_gemv_bytes(mt, nt), "right_gemv")):
+    gb = gemv_bytes(mt, nt)
+    for name, key in (("bd_fused_gemv", "fused_gemv"), ("bd_left_gemv", "left_gemv"),
+                      ("bd_right_gemv", "right_gemv")):
+        nbytes = gb[name]
+        if kt[f"{key}_launches"] == 0:
+            continue
         t = kt[f"{key}_ms"] / 1e3
         nl = max(kt[f"{key}_launches"], 1.0)
         gemv[name] = {"achieved_GBs": nbytes / t / 1e9 if t > 0 else None,
@@ -1020,9 +1057,10 @@ def main() -> int:
                                f"over {args.steps} extra steps ({ms_profiled:.0f} ms per SVD "
                                f"in those steps)"},
         "kernels": dict(gemv, bidiag_stage={
-            "achieved_GBs": 16.0 * S(m, n) / t_bd / 1e9,
-            "algorithmic_bytes_per_svd": 16 * S(m, n),
-            "note": "two FP64 reads of the trailing matrix per step"}),
+            "achieved_GBs": sum(gb.values()) / t_bd / 1e9,
+            "algorithmic_bytes_per_svd": sum(gb.values()),
+            "note": "one FP64 read of the trailing matrix per one-pass step, two per "
+                    "two-pass step"}),
         "stages_ms": {k: stage[k] for k in ddcsvd.TIMING_KEYS[:6]},
     }
     if not args.no_check:

@@ -31,6 +31,7 @@
 // takes the partial sums of the next panel's first left GEMV while each
 // tile is in registers, so that step launches no left GEMV.
 #include <algorithm>
+#include <cstdlib>
 
 #include "ddc_common.h"
 
@@ -637,26 +638,280 @@ __global__ __launch_bounds__(256) void bd_right_gemv(StepArgs<TS> a, int nrb, in
   }
 }
 
-// right_fin: one row in [k+1, m) per lane, FIN rows per group of four
-// waves; lane i of the grid also stores entry k+1+i of the right reflector
-// (the grid covers max(m, n) - k - 1 rows).  The four waves share the
-// loads of a row: wave w takes the panel columns s = 8w .. 8w+7 and the
-// chunk partials q = w, w + 4, ..., so that every load of the step is
-// issued at once with 16 + 32 loads per lane (one wave per group took 100
-// and a second round trip above 32 chunks); the partial sums meet in LDS.
+// ---------------------------------------------------------------------------
+// One pass over the trailing matrix per column step (m - k <= FG_ROWS).
+//
+// The two GEMVs of a step read the same block A0[k:m, k+1:n]: the left one
+// for g_j = A0[:, j]^T c, the right one for h = A0 r, whose r_j needs only
+// g_j and column j's panel terms.  A work-group that holds a whole column
+// can therefore finish both with one read (the BLAS 2.5 reorganisation of
+// Howell, Demmel, Fulton, Hammarling and Marmol, ACM TOMS 34(3), 2008):
+// per column it reduces g_j over all rows, forms y_t[j] and r_j as
+// bd_right_gemv's prologue does, and adds A0[:, j] r_j to its rows' partial
+// h.  Work-group w takes the columns k+1+w, k+1+w+G, ...; each column
+// stays in registers (RPT rows per thread) from its load to its last use,
+// and the next D-1 columns load meanwhile (a ring of D columns, indexed
+// statically by unrolling the column loop D times).  Two barriers per
+// column: the per-wave partial g is complete; r_j is known.  Work-group
+// w's partial h (all rows) goes to Rpart row w, its partial sums over its
+// columns of |r|^2, y_s^T r and v_s^T r to Rdot row w and rnorm[w], as
+// chunk w of the right GEMV would.  The panel dots u_s^T c and x_s^T c
+// (s < t) arrive as the previous bd_right_fin's per-group partials (nldot
+// rows of Ldot); the reflector of column k is stored by slices of rows
+// across the groups.
+constexpr int FG_T = 512;               // threads per work-group (256 registers each)
+constexpr int FG_ROWS = FG_T * 16;      // rows k..m-1 held: m - k <= 8192
+constexpr int FG_GMAX = 256;            // work-groups (one per CU)
+constexpr int FG_MAXCOL = 512;          // columns per work-group: n - k - 1 <= 256 x 512
+
+// Buffer loads: word 3 of a raw buffer resource on gfx9 targets (32-bit
+// data format, no swizzle); out-of-range offsets read zero.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+template <typename TS>
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, int voff);
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, int voff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<float>(__amdgpu_buffer_rsrc_t r, int voff) {
+  return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+}
+
+// Wave sum by DPP (row-local butterflies, then the row broadcasts of gfx9),
+// in a fixed order; every lane gets lane 63's total.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_take<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]: pairs
+  v += dpp_take<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]: quads
+  v += dpp_take<0x141, 0xF>(v);  // row_half_mirror: eights
+  v += dpp_take<0x140, 0xF>(v);  // row_mirror: rows of 16
+  v += dpp_take<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_take<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3: lane 63 has the total
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <typename TS, int RPT, int D>
+__global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nldot) {
+  constexpr int NW = FG_T / 64;
+  __shared__ double red[2][NW + 1];   // per-wave partial g and A0[k, j], by column parity
+  __shared__ double dsp[NW][2 * NB];  // partial panel dots
+  __shared__ double cvl[FG_T * RPT];  // c on rows k..k + FG_T RPT - 1
+  __shared__ double yst[FG_MAXCOL];   // y_t[j], r_j of the group's columns (stored
+  __shared__ double rst[FG_MAXCOL];   // after the loop: no store among its loads)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = a.k, t = a.t, m = a.m, n = a.n;
+  const int G = (int)gridDim.x, w = (int)blockIdx.x;
+  const int ncol = (n - k - 1 - w + G - 1) / G;  // columns of this group (>= 1)
+
+  // Per-step values first, every load unconditional and straight-line (a
+  // load under a branch, or a wait inside a loop, made the compiler wait
+  // for every load in flight): c on this thread's rows, the partial norms
+  // of c, the panel dots' partial rows, U[k, :] and X[k, :].  Then the
+  // first D-1 columns.
+  double cv[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) cv[q] = a.cvec[min(k + tid + FG_T * q, m - 1)];
+  constexpr int NCN = 2, NLD = 16;  // one batch; the rest (wide shapes) in loops
+  double cnb[NCN], ldb[NLD];
+#pragma unroll
+  for (int u = 0; u < NCN; ++u) cnb[u] = a.cnorm[min(lane + 64 * u, a.ncn - 1)];
+  const int dl = lane < 2 * t ? lane : 0;
+#pragma unroll
+  for (int u = 0; u < NLD; ++u)
+    ldb[u] = a.Ldot[(int64_t)min(wave + NW * u, max(nldot - 1, 0)) * (2 * NB) + dl];
+  const int sl = min(lane, max(t - 1, 0));
+  const double ukr = (double)a.A[k + (int64_t)(a.p + sl) * a.lda];
+  const double xkr = a.X[k + (int64_t)sl * a.lda];
+  const double ck = a.cvec[k];
+
+  // Column i of this group through buffer loads: the resource (scalar)
+  // ends at row m, so rows past it read zero, and is empty for i >= ncol.
+  // Every load is issued unconditionally, so that the compiler can wait
+  // for one ring slot while the later ones stay in flight.
+  const int voff = (k + tid) * (int)sizeof(TS);
+  auto fetch = [&](int i, double (&v)[RPT], double& yv, double& vv) {
+    const bool ok = i < ncol;
+    const int j = ok ? k + 1 + w + G * i : k + 1;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<TS*>(a.A + (int64_t)j * a.lda), 0, ok ? m * (int)sizeof(TS) : 0,
+        BUF_RSRC_WORD3);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) v[q] = buf_load<TS>(rs, voff + q * FG_T * (int)sizeof(TS));
+    // Y[j, s], V[j, s] on lane s < t
+    yv = a.Y[j + (int64_t)sl * a.ldy];
+    vv = a.V[j + (int64_t)sl * a.ldy];
+  };
+
+  double ring[D][RPT];
+  double ryv[D], rvv[D];
+#pragma unroll
+  for (int u = 0; u < D - 1; ++u) fetch(u, ring[u], ryv[u], rvv[u]);
+
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) cvl[tid + FG_T * q] = k + tid + FG_T * q < m ? cv[q] : 0.0;
+  // the reflector of c, formed by every wave alike (same partials, same order)
+  double cn = 0.0;
+#pragma unroll
+  for (int u = 0; u < NCN; ++u)
+    if (lane + 64 * u < a.ncn) cn += cnb[u];
+  for (int i = lane + 64 * NCN; i < a.ncn; i += 64) cn += a.cnorm[i];
+  Refl R;
+  R.make(wave_total(cn), ck);
+  // partial panel dots: lane = dot index, wave = residue of the partial row
+  double d = 0.0;
+#pragma unroll
+  for (int u = 0; u < NLD; ++u)
+    if (wave + NW * u < nldot) d += ldb[u];
+  for (int q = wave + NW * NLD; q < nldot; q += NW) d += a.Ldot[(int64_t)q * (2 * NB) + dl];
+  dsp[wave][lane] = lane < 2 * t ? d : 0.0;
+  __syncthreads();  // dsp, cvl
+  // every wave's per-lane panel values (lane s < t)
+  double dsu = 0.0, dsx = 0.0;
+  if (lane < t) {
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+      dsu += dsp[q][lane];
+      dsx += dsp[q][t + lane];
+    }
+  }
+  const double ukl = lane < t ? ukr : 0.0, xkl = lane < t ? xkr : 0.0;
+  const double ukt = R.vec(ck, true);
+
+  double h[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) h[q] = 0.0;
+  double oy = 0.0, ov = 0.0;  // wave 0: owner partial sums of lane s
+  for (int i0 = 0; i0 < ncol; i0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int i = i0 + u;
+      if (i < ncol) {
+        // the column D-1 ahead into the slot freed by column i-1
+        fetch(i + D - 1, ring[(u + D - 1) % D], ryv[(u + D - 1) % D], rvv[(u + D - 1) % D]);
+        const double (&v)[RPT] = ring[u];
+        double gp = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) gp += v[q] * cvl[tid + FG_T * q];
+        // every wave forms the panel terms of column j (lanes s >= t add
+        // zero) and then r_j itself: one barrier per column, with the
+        // partial g double-buffered by column parity
+        const double yv = lane < t ? ryv[u] : 0.0, vv = lane < t ? rvv[u] : 0.0;
+        const double P = wave_total(yv * dsu + vv * dsx);
+        const double Q = wave_total(yv * ukl + vv * xkl);
+        gp = wave_total(gp);
+        const int pb = i & 1;
+        if (lane == 0) red[pb][wave] = gp;
+        if (tid == 0) red[pb][NW] = v[0];  // row k
+        __syncthreads();
+        double g = 0.0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) g += red[pb][q];
+        const double rk = red[pb][NW];
+        const double y = 2.0 * R.dot(g - P, rk - Q);
+        const double r = rk - Q - ukt * y;
+        if (wave == 0) {
+          if (lane == 0) {
+            yst[i] = y;
+            rst[i] = r;
+          }
+          // owner slots of lane s: y_s^T r (s < t), y_t^T r (s = t),
+          // |r|^2 (s = 63); v_s^T r (s < t)
+          oy += (lane < t ? ryv[u] : (lane == t ? y : (lane == 63 ? r : 0.0))) * r;
+          ov += (lane < t ? rvv[u] : 0.0) * r;
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) h[q] += v[q] * r;
+      }
+    }
+  }
+  __syncthreads();  // yst, rst
+  // y_t and r on the group's columns
+  for (int i = tid; i < ncol; i += FG_T) {
+    const int j = k + 1 + w + G * i;
+    a.Y[j + (int64_t)t * a.ldy] = yst[i];
+    a.rvec[j] = rst[i];
+  }
+  // the reflector of column k: rows k + w S .. in slices
+  {
+    const int S = (m - k + G - 1) / G;
+    for (int i = k + w * S + tid; i < min(m, k + (w + 1) * S); i += FG_T) {
+      const double u = R.vec(a.cvec[i], i == k);
+      a.A[i + (int64_t)k * a.lda] = (TS)u;
+      if (a.Ucopy) a.Ucopy[i + (int64_t)t * a.lda] = u;
+    }
+    if (w == 0 && tid == 0) a.alpha[k] = R.diag();
+  }
+  // partial h on rows k+1..m-1
+  double* out = a.Rpart + (int64_t)w * a.lda;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int i = k + tid + FG_T * q;
+    if (i >= k + 1 && i < m) out[i] = h[q];
+  }
+  if (wave == 0) {
+    double* Rd = a.Rdot + (int64_t)w * DSTRIDE;
+    if (lane <= t) Rd[lane] = oy;
+    if (lane < t) Rd[t + 1 + lane] = ov;
+    if (lane == 63) a.rnorm[w] = oy;
+  }
+}
+
+// Rows per thread of the one-pass GEMV for m - k rows, and its launch.
+template <typename TS>
+static void launch_fused_gemv(const StepArgs<TS>& a, int G, int nldot, bool clock,
+                              hipStream_t st) {
+  const int rows = a.m - a.k;
+  hipEvent_t ke0, ke1;
+  const bool ck = clock && kclock_on() && kclock_pair(KCLOCK_FUSED_GEMV, &ke0, &ke1) == 0;
+  auto go = [&](auto kern) {
+    if (ck)
+      hipExtLaunchKernelGGL(kern, dim3(G), dim3(FG_T), 0, st, ke0, ke1, 0, a, nldot);
+    else
+      hipLaunchKernelGGL(kern, dim3(G), dim3(FG_T), 0, st, a, nldot);
+  };
+  // ring depths: as many columns as the 256 registers hold without spills
+  if (rows > FG_T * 8) go(bd_fused_gemv<TS, 16, 3>);
+  else if (rows > FG_T * 4) go(bd_fused_gemv<TS, 8, 8>);
+  else if (rows > FG_T * 2) go(bd_fused_gemv<TS, 4, 8>);
+  else go(bd_fused_gemv<TS, 2, 8>);
+}
+
+// right_fin: one row in [k+1, m) per lane, FIN rows per group of NWF (4
+// or 8) waves; lane i of the grid also stores entry k+1+i of the right
+// reflector (the grid covers max(m, n) - k - 1 rows).  The waves share the
+// loads of a row: wave w takes the panel columns s = PW w .. PW w + PW-1
+// and the chunk partials q = w, w + NWF, ..., so that every load of the
+// step is issued at once (one wave per group took 100 loads per lane and a
+// second round trip above 32 chunks); the partial sums meet in LDS.
 // noright: the row of M has a single entry, taken as beta with no
 // reflection (bidiag.c:125-129).
-template <typename TS>
-__global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright) {
-  constexpr int PW = NB / 4;  // panel columns per wave
-  constexpr int PB = 16;      // chunk partials per wave and batch
+//
+// dots: also leave, per group, the next step's panel dots over its rows,
+// u_s^T c and x_s^T c for s <= t (c the new column), in Ldot row
+// blockIdx.x at [s] and [t + 1 + s], for bd_fused_gemv.  PB: chunk
+// partials per wave and batch (32 for the fused GEMV's up to 256).
+template <typename TS, int PB, int NWF>
+__global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nccR, int noright,
+                                                          int dots) {
+  constexpr int PW = NB / NWF;  // panel columns per wave
   __shared__ double dyv[DSTRIDE];  // y_s^T v (s <= t), v_s^T v (s < t)
   __shared__ double yk1[NB];       // Y[k+1, s], s <= t
   __shared__ double vk1[NB];       // V[k+1, s], s < t; [t]: v[k+1]
-  __shared__ double hp[4][FIN];    // per wave: partial A0[row, k+1:n] r
-  __shared__ double dp[4][FIN];    // per wave: partial panel dots with r
-  __shared__ double xp[4][FIN];    // per wave: panel terms of x
-  __shared__ double cp[4][FIN];    // per wave: panel terms of c
+  __shared__ double hp[NWF][FIN];  // per wave: partial A0[row, k+1:n] r
+  __shared__ double dp[NWF][FIN];  // per wave: partial panel dots with r
+  __shared__ double xp[NWF][FIN];  // per wave: panel terms of x
+  __shared__ double cp[NWF][FIN];  // per wave: panel terms of c
+  __shared__ double cnl[FIN];      // dots: the new c on the group's rows
   __shared__ double rfl[3];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k = a.k, t = a.t;
@@ -684,7 +939,7 @@ __global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR
   double pd[PB], ph[PB];
 #pragma unroll
   for (int u = 0; u < PB; ++u) {
-    const int q = min(wave + 4 * u, ncl - 1);
+    const int q = min(wave + NWF * u, ncl - 1);
     pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
     ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
   }
@@ -710,20 +965,20 @@ __global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR
   if (!noright) {
 #pragma unroll
     for (int u = 0; u < PB; ++u)
-      if (wave + 4 * u < nccR) {
+      if (wave + NWF * u < nccR) {
         ds += pd[u];
         h += ph[u];
       }
-    for (int q0 = 4 * PB; q0 < nccR; q0 += 4 * PB) {
+    for (int q0 = NWF * PB; q0 < nccR; q0 += NWF * PB) {
 #pragma unroll
       for (int u = 0; u < PB; ++u) {
-        const int q = min(q0 + wave + 4 * u, nccR - 1);
+        const int q = min(q0 + wave + NWF * u, nccR - 1);
         pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
         ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
       }
 #pragma unroll
       for (int u = 0; u < PB; ++u)
-        if (q0 + wave + 4 * u < nccR) {
+        if (q0 + wave + NWF * u < nccR) {
           ds += pd[u];
           h += ph[u];
         }
@@ -748,7 +1003,12 @@ __global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR
     if (lane < t) vk1[lane] = vk;
   }
   __syncthreads();
-  if (wave == 0 && dotr) dyv[lane] = R.dot(dp[0][lane] + dp[1][lane] + dp[2][lane] + dp[3][lane], w1);
+  if (wave == 0 && dotr) {
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWF; ++q) d += dp[q][lane];
+    dyv[lane] = R.dot(d, w1);
+  }
   __syncthreads();
 
   // this wave's panel terms of x and c
@@ -764,24 +1024,34 @@ __global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR
   xp[wave][lane] = xc;
   cp[wave][lane] = cc;
   __syncthreads();
-  if (wave != 0) return;
-
-  double cc2 = 0.0;
+  double xrow = 0.0;  // dots: x_t on this lane's row (wave 0)
+  if (wave == 0) {
+  double cc2 = 0.0, cnew = 0.0;
   if (rv) {
-    double x = R.dot(hp[0][lane] + hp[1][lane] + hp[2][lane] + hp[3][lane], c);  // 0 if noright
-    x -= xp[0][lane] + xp[1][lane] + xp[2][lane] + xp[3][lane];
+    double hs = 0.0, xs2 = 0.0, cs2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < NWF; ++q) {
+      hs += hp[q][lane];
+      xs2 += xp[q][lane];
+      cs2 += cp[q][lane];
+    }
+    double x = R.dot(hs, c);  // 0 if noright
+    x -= xs2;
     if (!noright) {
       x -= ut * dyv[t];
       x *= 2.0;
     }
     a.X[row + (int64_t)t * a.lda] = x;
+    xrow = x;
     if (next) {
-      c -= cp[0][lane] + cp[1][lane] + cp[2][lane] + cp[3][lane];
+      c -= cs2;
       c -= ut * yk1[t] + x * vk1[t];
       a.cvec[row] = c;
       cc2 = c * c;
+      cnew = c;
     }
   }
+  if (dots) cnl[lane] = cnew;
   const double s = wave_sum(cc2);
   if (lane == 0) a.cnorm[blockIdx.x] = s;
   // the right reflector to row k of A and to column t of V (no work-group
@@ -798,6 +1068,33 @@ __global__ __launch_bounds__(4 * FIN) void bd_right_fin(StepArgs<TS> a, int nccR
     a.beta[k] = a.rvec[k + 1];
     a.A[k + (int64_t)(k + 1) * a.lda] = (TS)0;
     a.V[(k + 1) + (int64_t)t * a.ldy] = 0.0;
+  }
+  }
+  if (!dots) return;
+  __syncthreads();
+  // the next step's panel dots over this group's rows (wave w: s = 8w ..
+  // 8w+7 below t from us, xs; wave 0 also s = t from u_t and x_t)
+  const double cl = cnl[lane];
+  double gd[2 * PW];
+#pragma unroll
+  for (int i = 0; i < PW; ++i) {
+    const bool in = s0 + i < t;
+    gd[i] = in ? us[i] * cl : 0.0;
+    gd[PW + i] = in ? xs[i] * cl : 0.0;
+  }
+  const double tot = reduce_scatter<double, 2 * PW>(gd);  // lane l: slot l / (64 / 2 PW)
+  double* Ld = a.Ldot + (int64_t)blockIdx.x * (2 * NB);
+  constexpr int LPS = 64 / (2 * PW);
+  if ((lane & (LPS - 1)) == 0) {
+    const int slot = lane / LPS, i = slot % PW, sx = s0 + i;
+    if (sx < t) Ld[slot < PW ? sx : t + 1 + sx] = tot;
+  }
+  if (wave == 0) {
+    const double du = wave_sum(ut * cl), dx = wave_sum(xrow * cl);
+    if (lane == 0) {
+      Ld[t] = du;
+      Ld[2 * t + 1] = dx;
+    }
   }
 }
 
@@ -877,10 +1174,11 @@ static BidiagBufs bidiag_dims(int m, int n, int lda) {
   d.ldy = (int)round_up(n, 16);
   // left GEMV partials: one per 2048-row chunk, or one per 128-row tile of
   // the trailing update that carries the next column's left GEMV
-  d.nchL_max = (int)std::max(ceil_div(lda, RC), ceil_div(lda, TRAIL_TILE)) + 1;
-  d.nccR_max = (int)ceil_div(n, 64) + 1;  // 64- or 128-column chunks
+  d.nchL_max = (int)std::max({ceil_div(lda, RC), ceil_div(lda, TRAIL_TILE),
+                              ceil_div(std::max(lda, n), FIN)}) + 1;  // also right_fin's dots
+  d.nccR_max = (int)std::max<int64_t>(ceil_div(n, 64), FG_GMAX) + 1;  // chunks or fused groups
   d.ncn_max = (int)std::max(ceil_div(m, 256), ceil_div(std::max(m, n), FIN)) + 1;
-  d.nrn_max = (int)ceil_div(n, FIN) + 1;
+  d.nrn_max = (int)std::max<int64_t>(ceil_div(n, FIN), FG_GMAX) + 1;
   return d;
 }
 
@@ -993,6 +1291,11 @@ static int launch_right_gemv(const StepArgs<TS>& a, int nchL, bool clock, hipStr
   return ncc;
 }
 
+static bool fused_gemv_on() {
+  static const bool on = getenv("DDCSVD_TWO_PASS") == nullptr;
+  return on;
+}
+
 template <typename TS>
 static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* beta, int upd,
                         Arena& ws, hipStream_t st) {
@@ -1048,7 +1351,15 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
   DDC_LAUNCHED();
 
   const int mn = std::min(m, n);
+  // column steps whose two GEMVs are one pass (bd_fused_gemv): the plain
+  // steps (not the last two columns or the wide last row) on at most
+  // FG_ROWS rows
+  auto one_pass = [&](int kk) {
+    return fused_gemv_on() && kk < n - 2 && !(m < n && kk >= m - 1) && m - kk <= FG_ROWS &&
+           n - kk - 1 <= FG_GMAX * FG_MAXCOL;
+  };
   int fusedL = 0;  // tiles of left GEMV partials the trailing update left for step k, or 0
+  int nfr_prev = 0;  // groups of the previous bd_right_fin (rows of its panel dots)
   for (int k = 0; k < mn; ++k) {
     a.k = k;
     a.p = (k / NB) * NB;
@@ -1069,6 +1380,26 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
       DDC_LAUNCHED();
       break;
     }
+    // right_fin's grid covers the rows k+1..m-1 and, for the right
+    // reflector, the columns k+1..n-1
+    const int nfr = (int)ceil_div(std::max(m, n) - k - 1, FIN);
+    // the next step of this panel, if one pass, takes its panel dots from
+    // this step's bd_right_fin
+    const int dots = (a.t + 1 < NB && one_pass(k + 1)) ? 1 : 0;
+    if (one_pass(k)) {
+      const int G = std::min(FG_GMAX, n - k - 1);
+      launch_fused_gemv(a, G, a.t > 0 ? nfr_prev : 0, true, st);
+      DDC_LAUNCHED();
+      a.nrn = G;
+      // the one-pass groups' partials: up to 256, one batch of 32 per wave of 8
+      if (G > 4 * 16)
+        hipLaunchKernelGGL((bd_right_fin<TS, 32, 8>), dim3(nfr), dim3(8 * FIN), 0, st, a, G, 0,
+                           dots);
+      else
+        hipLaunchKernelGGL((bd_right_fin<TS, 16, 4>), dim3(nfr), dim3(4 * FIN), 0, st, a, G, 0,
+                           dots);
+      DDC_LAUNCHED();
+    } else {
     // left reflector of column k and the left GEMV
     const int rb0 = (k / 64) * 64;
     const int ncb = (int)ceil_div(n - k - 1, CBL);
@@ -1090,24 +1421,24 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
     DDC_LAUNCHED();
     // row k of M (the left finalization), the right reflector of row k and
     // the right GEMV
-    // right_fin's grid covers the rows k+1..m-1 and, for the right
-    // reflector, the columns k+1..n-1
-    const int nfr = (int)ceil_div(std::max(m, n) - k - 1, FIN);
     if (k == n - 2) {
       const int nfl = (int)ceil_div(n - k - 1, FIN);
       hipLaunchKernelGGL(bd_left_fin<TS>, dim3(nfl), dim3(FIN), 0, st, a, nchL);
       DDC_LAUNCHED();
       a.nrn = nfl;
-      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(4 * FIN), 0, st, a, 0, 1);
+      hipLaunchKernelGGL((bd_right_fin<TS, 16, 4>), dim3(nfr), dim3(4 * FIN), 0, st, a, 0, 1, 0);
       DDC_LAUNCHED();
     } else {
       const int ncc = launch_right_gemv(a, nchL, true, st);
       DDC_LAUNCHED();
       a.nrn = ncc;
-      hipLaunchKernelGGL(bd_right_fin<TS>, dim3(nfr), dim3(4 * FIN), 0, st, a, ncc, 0);
+      hipLaunchKernelGGL((bd_right_fin<TS, 16, 4>), dim3(nfr), dim3(4 * FIN), 0, st, a, ncc, 0,
+                         dots);
       DDC_LAUNCHED();
     }
+    }
     a.ncn = nfr;
+    nfr_prev = nfr;
     // blocked trailing update at the end of a full panel
     if (a.t == NB - 1 && a.p + NB < n) {
       const int r0 = a.p + NB;
@@ -1117,7 +1448,7 @@ static int bidiag_drive(int m, int n, TS* A, int lda, double* alpha, double* bet
         // GEMV (column r0 of the updated matrix is c, from bd_right_fin)
         // when that step is a plain one: one pass over the trailing matrix
         // less per panel.
-        const bool fuse = r0 < n - 1 && !(m < n && r0 == m - 1);
+        const bool fuse = !one_pass(r0) && r0 < n - 1 && !(m < n && r0 == m - 1);
         DDC_TRY(dgemm_trail64(m - r0, n - r0, UX + r0, lda, a.Y + r0, d.ldy, C, lda,
                               fuse ? a.cvec + r0 : nullptr, a.Lpart + r0, d.ldy, 1, st));
         if (fuse) fusedL = (int)ceil_div(m - r0, TRAIL_TILE);
@@ -1159,6 +1490,55 @@ int bidiag_run(int m, int n, double* A, int lda, double* alpha, double* beta, Ar
 // alpha with their diagonals).  The column of step 0 is c for every step and
 // no other kernel of the step runs, so a counter pass sees the left GEMV's
 // own traffic for the trailing blocks A[k:m, k+1:n].
+// Measurement: `steps` one-pass GEMV launches, at column steps k =
+// 0..steps-1, on the panel-start matrix A (m >= n, m <= FG_ROWS; A is
+// overwritten with reflectors, alpha with their diagonals).  The column of
+// step 0 is c for every step, the panel dots are zero and no other kernel
+// of the step runs, so a counter pass sees the kernel's own traffic for
+// the trailing blocks A[k:m, k+1:n] (one read each).
+int bidiag_fused_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps,
+                           Arena& ws, hipStream_t st) {
+  if (m < n || n < 3 || m > FG_ROWS || lda < m || (lda % 2) != 0 || steps < 1 || steps > n - 2)
+    return DDC_E_ARG;
+  BidiagBufs d = bidiag_dims(m, n, lda);
+  StepArgs<double> a;
+  a.m = m;
+  a.n = n;
+  a.lda = lda;
+  a.ldy = d.ldy;
+  a.A = A;
+  a.alpha = alpha;
+  a.beta = nullptr;
+  a.Y = ws.take<double>((size_t)d.ldy * NB * 2);
+  a.V = a.Y + (size_t)d.ldy * NB;
+  double* UX = ws.take<double>((size_t)lda * NB * 2);
+  a.X = UX ? UX + (size_t)lda * NB : nullptr;
+  a.Ucopy = UX;
+  a.cvec = ws.take<double>(lda);
+  a.rvec = ws.take<double>(d.ldy);
+  a.Lpart = ws.take<double>((size_t)d.nchL_max * d.ldy);
+  a.Rpart = ws.take<double>((size_t)d.nccR_max * lda);
+  a.Ldot = ws.take<double>((size_t)d.nchL_max * 2 * NB);
+  a.Rdot = ws.take<double>((size_t)d.nccR_max * DSTRIDE);
+  a.cnorm = ws.take<double>(d.ncn_max);
+  a.rnorm = ws.take<double>(d.nrn_max);
+  if (!a.rnorm) return DDC_E_NOMEM;
+  DDC_HIP(hipMemsetAsync(a.Y, 0, sizeof(double) * (size_t)d.ldy * NB * 2, st));
+  DDC_HIP(hipMemsetAsync(a.X, 0, sizeof(double) * (size_t)lda * NB, st));
+  a.ncn = (int)ceil_div(m, 256);
+  a.nrn = 0;
+  hipLaunchKernelGGL(bd_init<double>, dim3(a.ncn), dim3(256), 0, st, m, A, a.cvec, a.cnorm);
+  DDC_LAUNCHED();
+  for (int k = 0; k < steps; ++k) {
+    a.k = k;
+    a.p = (k / NB) * NB;
+    a.t = k - a.p;
+    launch_fused_gemv(a, std::min(FG_GMAX, n - k - 1), 0, false, st);
+    DDC_LAUNCHED();
+  }
+  return DDC_OK;
+}
+
 int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps, Arena& ws,
                           hipStream_t st) {
   if (m < n || n < 2 || lda < m || (lda % 2) != 0 || steps < 1 || steps > n - 1)

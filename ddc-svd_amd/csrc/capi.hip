@@ -21,7 +21,7 @@ namespace ddc {
 namespace {
 std::mutex g_mu;
 char g_err[512] = "";
-constexpr int kNumTimes = 10;
+constexpr int kNumTimes = 12;
 double g_times[kNumTimes] = {};
 
 // Per-launch kernel clock (see kclock_on in ddc_common.h).
@@ -109,16 +109,17 @@ struct KClockScope {
 };
 
 // After the stream has drained: milliseconds and launch counts per class
-// into g_times[6..9].
+// into g_times[6..9] (left, right GEMV) and [10..11] (one-pass GEMV).
 static void kclock_collect() {
+  static const int slot_ms[KCLOCK_CLASSES] = {6, 7, 10}, slot_n[KCLOCK_CLASSES] = {8, 9, 11};
   for (int c = 0; c < KCLOCK_CLASSES; ++c) {
     double ms = 0.0;
     for (auto& pr : g_kc.spans[c]) {
       float t = 0.f;
       if (hipEventElapsedTime(&t, g_kc.pool[pr.first], g_kc.pool[pr.second]) == hipSuccess) ms += t;
     }
-    g_times[6 + c] = g_kc.active && !g_kc.failed ? ms : 0.0;
-    g_times[8 + c] = g_kc.active && !g_kc.failed ? (double)g_kc.spans[c].size() : 0.0;
+    g_times[slot_ms[c]] = g_kc.active && !g_kc.failed ? ms : 0.0;
+    g_times[slot_n[c]] = g_kc.active && !g_kc.failed ? (double)g_kc.spans[c].size() : 0.0;
   }
   g_kc.active = false;
 }
@@ -519,11 +520,17 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
 }
 
 // Counter-collection workload: a padded copy of dA, then `steps` left-GEMV
-// (right: right-GEMV) launches of its bidiagonalization and nothing else.
-static int gemv_only_dev(bool right, int m, int n, const double* dA, int lda, int steps,
+// (kind 1: right-GEMV, kind 2: one-pass GEMV) launches of its
+// bidiagonalization and nothing else.
+static int gemv_only_dev(int kind, int m, int n, const double* dA, int lda, int steps,
                          hipStream_t user) {
   DDC_TRY(check_dims(m, n));
-  if (!right && (!dA || m < n || n < 2 || lda < m || steps < 1 || steps > n - 1)) {
+  const bool right = kind == 1;
+  if (kind == 2 && (!dA || m < n || n < 3 || m > 8192 || lda < m || steps < 1 || steps > n - 2)) {
+    set_err("invalid arguments (need 8192 >= m >= n >= 3, lda >= m, 1 <= steps <= n-2)");
+    return DDC_E_ARG;
+  }
+  if (kind == 0 && (!dA || m < n || n < 2 || lda < m || steps < 1 || steps > n - 1)) {
     set_err("invalid arguments (need m >= n >= 2, lda >= m, 1 <= steps <= n-1)");
     return DDC_E_ARG;
   }
@@ -545,7 +552,9 @@ static int gemv_only_dev(bool right, int m, int n, const double* dA, int lda, in
   DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
   DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, dA, sizeof(double) * lda,
                            sizeof(double) * m, n, hipMemcpyDeviceToDevice, st));
-  if (right)
+  if (kind == 2)
+    DDC_TRY(bidiag_fused_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
+  else if (right)
     DDC_TRY(bidiag_right_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
   else
     DDC_TRY(bidiag_left_gemv_only(m, n, Ap, lda_p, da, steps, ws, st));
@@ -880,14 +889,21 @@ int svd_gpu_profile_left_gemv(int m, int n, const double* dA, int lda, int steps
                               void* stream) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_err[0] = 0;
-  return gemv_only_dev(false, m, n, dA, lda, steps, (hipStream_t)stream);
+  return gemv_only_dev(0, m, n, dA, lda, steps, (hipStream_t)stream);
 }
 
 int svd_gpu_profile_right_gemv(int m, int n, const double* dA, int lda, int steps,
                                void* stream) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_err[0] = 0;
-  return gemv_only_dev(true, m, n, dA, lda, steps, (hipStream_t)stream);
+  return gemv_only_dev(1, m, n, dA, lda, steps, (hipStream_t)stream);
+}
+
+int svd_gpu_profile_fused_gemv(int m, int n, const double* dA, int lda, int steps,
+                               void* stream) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err[0] = 0;
+  return gemv_only_dev(2, m, n, dA, lda, steps, (hipStream_t)stream);
 }
 
 int svd_gpu_profile_kernels(int on) {

@@ -35,7 +35,7 @@ bool kclock_on();
 int kclock_mark(hipStream_t st);
 void kclock_span(int cls, int a, int b);
 int kclock_pair(int cls, hipEvent_t* e0, hipEvent_t* e1);
-enum { KCLOCK_LEFT_GEMV = 0, KCLOCK_RIGHT_GEMV = 1, KCLOCK_CLASSES = 2 };
+enum { KCLOCK_LEFT_GEMV = 0, KCLOCK_RIGHT_GEMV = 1, KCLOCK_FUSED_GEMV = 2, KCLOCK_CLASSES = 3 };
 
 // Message returned by svd_gpu_last_error(); clear_error() empties it.
 void report_error(const char* msg);
@@ -95,6 +95,9 @@ void bidiag_plan(ArenaPlan& p, int m, int n, int lda);
 // Measurement: `steps` left-GEMV launches alone (A and alpha overwritten).
 int bidiag_left_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps, Arena& ws,
                           hipStream_t st);
+// Measurement: `steps` one-pass GEMV launches alone (m <= 8192).
+int bidiag_fused_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps,
+                           Arena& ws, hipStream_t st);
 // Measurement: `steps` right-GEMV launches (after one left GEMV at k = 0).
 int bidiag_right_gemv_only(int m, int n, double* A, int lda, double* alpha, int steps,
                            Arena& ws, hipStream_t st);

@@ -85,6 +85,8 @@ def load():
         lib.svd_gpu_profile_left_gemv.restype = i
         lib.svd_gpu_profile_right_gemv.argtypes = [i, i, v, i, i, v]
         lib.svd_gpu_profile_right_gemv.restype = i
+        lib.svd_gpu_profile_fused_gemv.argtypes = [i, i, v, i, i, v]
+        lib.svd_gpu_profile_fused_gemv.restype = i
         lib.svd_gpu_last_error.argtypes = []
         lib.svd_gpu_last_error.restype = ctypes.c_char_p
         c = ctypes.c_char
@@ -109,7 +111,8 @@ def last_error() -> str:
 
 
 TIMING_KEYS = ["copy_in", "bidiag", "values", "vectors", "backtransform", "copy_out",
-               "left_gemv_ms", "right_gemv_ms", "left_gemv_launches", "right_gemv_launches"]
+               "left_gemv_ms", "right_gemv_ms", "left_gemv_launches", "right_gemv_launches",
+               "fused_gemv_ms", "fused_gemv_launches"]
 
 
 def last_timings() -> dict:
@@ -140,6 +143,15 @@ def profile_right_gemv(A, steps: int, stream=None) -> None:
     m, n = A.shape
     lda = _colmajor_ld(A, "A")
     _check(load().svd_gpu_profile_right_gemv(m, n, ctypes.c_void_p(A.data_ptr()), lda, steps,
+                                             ctypes.c_void_p(stream)))
+
+
+def profile_fused_gemv(A, steps: int, stream=None) -> None:
+    """svd_gpu_profile_fused_gemv on a column-major float64 CUDA matrix: the
+    first `steps` one-pass GEMV launches of its bidiagonalization, alone."""
+    m, n = A.shape
+    lda = _colmajor_ld(A, "A")
+    _check(load().svd_gpu_profile_fused_gemv(m, n, ctypes.c_void_p(A.data_ptr()), lda, steps,
                                              ctypes.c_void_p(stream)))
 
 

@@ -253,8 +253,10 @@ int svd_gpu_dgemm(char transa, char transb, int M, int N, int K, double alpha,
  * [2] singular values, [3] vectors, [4] back-transformation, [5] copy-out.
  * With kernel profiling on (svd_gpu_profile_kernels): [6] / [7] summed
  * durations of the left / right GEMV launches of the bidiagonalization and
- * [8] / [9] their launch counts (zeros otherwise).  Returns the number of
- * entries written (at most count, at most 10). */
+ * [8] / [9] their launch counts, [10] / [11] those of the one-pass GEMV
+ * (both products of a column step in one read of the trailing matrix)
+ * (zeros otherwise).  Returns the number of entries written (at most
+ * count, at most 12). */
 int svd_gpu_last_timings(double *ms, int count);
 
 /* Measurement hook (no reference counterpart): on != 0 brackets every GEMV
@@ -276,6 +278,16 @@ int svd_gpu_profile_left_gemv(int m, int n, const double *dA, int lda, int steps
  * with no other kernel in between: each launch streams the trailing block of
  * its column step, as in the SVD.  Workspace only; results are discarded. */
 int svd_gpu_profile_right_gemv(int m, int n, const double *dA, int lda, int steps,
+                               void *stream);
+
+/* Measurement hook (no reference counterpart): the first `steps` launches
+ * (1 <= steps <= n-2) of the one-pass GEMV (both products of a column step
+ * in one read of the trailing matrix) of the bidiagonalization of a copy of
+ * the m x n device matrix dA (8192 >= m >= n >= 3), with no other kernel of
+ * those column steps in between, for hardware-counter passes over that
+ * kernel alone.  The launches read and write only library workspace;
+ * results are discarded. */
+int svd_gpu_profile_fused_gemv(int m, int n, const double *dA, int lda, int steps,
                                void *stream);
 
 /* Human-readable description of the last failure (empty if none). */

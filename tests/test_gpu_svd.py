@@ -451,6 +451,26 @@ def test_profile_right_gemv_hook(ddcsvd):
                                                                    device="cuda")), 1)
 
 
+def test_profile_fused_gemv_hook(ddcsvd):
+    """The one-pass GEMV counter hook runs on a workspace copy (the caller's
+    matrix is unchanged) and rejects step counts outside 1..n-2 and shapes
+    with m < n, n < 3 or m > 8192."""
+    import torch
+    A = gen.normal_matrix(700, 300, 37)
+    At = ddcsvd.col_major(torch.from_numpy(A).to("cuda"))
+    gpu_case("profile_fused_gemv 700x300, 298 steps")
+    ddcsvd.profile_fused_gemv(At, 298)
+    torch.cuda.synchronize()
+    assert np.array_equal(At.cpu().numpy(), A)
+    for steps in (0, 299):
+        with pytest.raises(ddcsvd.SvdGpuError):
+            ddcsvd.profile_fused_gemv(At, steps)
+    for shape in ((3, 5), (4, 2), (8193, 3)):
+        with pytest.raises(ddcsvd.SvdGpuError):
+            ddcsvd.profile_fused_gemv(ddcsvd.col_major(torch.zeros(shape, dtype=torch.float64,
+                                                                   device="cuda")), 1)
+
+
 @pytest.mark.parametrize("shape", [(4100, 1100), (1100, 4100)])
 def test_host_buffers_staged_copies_match_device_path(ddcsvd, shape):
     """Host-buffer calls above 4 MiB move their data through the library's
