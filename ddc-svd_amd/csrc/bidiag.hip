@@ -687,23 +687,66 @@ __device__ __forceinline__ double dpp_take(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
-__device__ __forceinline__ double wave_total(double v) {
-  v += dpp_take<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]: pairs
-  v += dpp_take<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]: quads
-  v += dpp_take<0x141, 0xF>(v);  // row_half_mirror: eights
-  v += dpp_take<0x140, 0xF>(v);  // row_mirror: rows of 16
-  v += dpp_take<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
-  v += dpp_take<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3: lane 63 has the total
+// (every lane written: no old value to set)
+template <int CTRL>
+__device__ __forceinline__ double dpp_all(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane63(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
   const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, 63);
   const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_all<0xB1>(v);         // quad_perm [1,0,3,2]: pairs
+  v += dpp_all<0x4E>(v);         // quad_perm [2,3,0,1]: quads
+  v += dpp_all<0x141>(v);        // row_half_mirror: eights
+  v += dpp_all<0x140>(v);        // row_mirror: rows of 16
+  v += dpp_take<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_take<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3: lane 63 has the total
+  return lane63(v);
+}
+// Three wave totals with their stages interleaved (independent chains).
+__device__ __forceinline__ void wave_total3(double& a, double& b, double& c) {
+  a += dpp_all<0xB1>(a);
+  b += dpp_all<0xB1>(b);
+  c += dpp_all<0xB1>(c);
+  a += dpp_all<0x4E>(a);
+  b += dpp_all<0x4E>(b);
+  c += dpp_all<0x4E>(c);
+  a += dpp_all<0x141>(a);
+  b += dpp_all<0x141>(b);
+  c += dpp_all<0x141>(c);
+  a += dpp_all<0x140>(a);
+  b += dpp_all<0x140>(b);
+  c += dpp_all<0x140>(c);
+  a += dpp_take<0x142, 0xA>(a);
+  b += dpp_take<0x142, 0xA>(b);
+  c += dpp_take<0x142, 0xA>(c);
+  a += dpp_take<0x143, 0xC>(a);
+  b += dpp_take<0x143, 0xC>(b);
+  c += dpp_take<0x143, 0xC>(c);
+  a = lane63(a);
+  b = lane63(b);
+  c = lane63(c);
+}
 
-template <typename TS, int RPT, int D>
+// num / d from ir = 1/d (0 for d = 0, giving 0): q0 = num ir, then one
+// correction q0 + (num - q0 d) ir with fused multiply-adds.
+__device__ __forceinline__ double rl_quot(double num, double d, double ir) {
+  const double q0 = num * ir;
+  return fma(fma(-q0, d, num), ir, q0);
+}
+
+template <typename TS, int RPT, int D, bool PAIR>
 __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nldot) {
   constexpr int NW = FG_T / 64;
-  __shared__ double red[2][NW + 1];   // per-wave partial g and A0[k, j], by column parity
+  // per column (pair) parity: partial g by wave, A0[k, j], P, Q
+  __shared__ double red[2][2 * NW + 6];
   __shared__ double dsp[NW][2 * NB];  // partial panel dots
   __shared__ double cvl[FG_T * RPT];  // c on rows k..k + FG_T RPT - 1
   __shared__ double yst[FG_MAXCOL];   // y_t[j], r_j of the group's columns (stored
@@ -730,6 +773,7 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
   for (int u = 0; u < NLD; ++u)
     ldb[u] = a.Ldot[(int64_t)min(wave + NW * u, max(nldot - 1, 0)) * (2 * NB) + dl];
   const int sl = min(lane, max(t - 1, 0));
+  const int slw = wave == 0 ? sl : 0;
   const double ukr = (double)a.A[k + (int64_t)(a.p + sl) * a.lda];
   const double xkr = a.X[k + (int64_t)sl * a.lda];
   const double ck = a.cvec[k];
@@ -747,15 +791,18 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
         BUF_RSRC_WORD3);
 #pragma unroll
     for (int q = 0; q < RPT; ++q) v[q] = buf_load<TS>(rs, voff + q * FG_T * (int)sizeof(TS));
-    // Y[j, s], V[j, s] on lane s < t
-    yv = a.Y[j + (int64_t)sl * a.ldy];
-    vv = a.V[j + (int64_t)sl * a.ldy];
+    // Y[j, s], V[j, s] on lane s < t of wave 0 (the other waves load one
+    // line, unused: every load stays unconditional)
+    yv = a.Y[j + (int64_t)slw * a.ldy];
+    vv = a.V[j + (int64_t)slw * a.ldy];
   };
 
+  // columns in flight ahead of the current one (pairs: the current pair)
+  constexpr int AHEAD = PAIR ? D - 2 : D - 1;
   double ring[D][RPT];
   double ryv[D], rvv[D];
 #pragma unroll
-  for (int u = 0; u < D - 1; ++u) fetch(u, ring[u], ryv[u], rvv[u]);
+  for (int u = 0; u < AHEAD; ++u) fetch(u, ring[u], ryv[u], rvv[u]);
 
 #pragma unroll
   for (int q = 0; q < RPT; ++q) cvl[tid + FG_T * q] = k + tid + FG_T * q < m ? cv[q] : 0.0;
@@ -786,11 +833,109 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
   }
   const double ukl = lane < t ? ukr : 0.0, xkl = lane < t ? xkr : 0.0;
   const double ukt = R.vec(ck, true);
+  // y_t[j] = 2 ((g - P) + s|c| (rk - Q)) / rl, as Refl::dot, by the
+  // reciprocal of rl and one FMA residual correction (Markstein): off the
+  // latency of a full division per column, with its rounding
+  const double snrm = R.sgn * R.nrm, irl = R.rl != 0.0 ? 1.0 / R.rl : 0.0;
 
   double h[RPT];
 #pragma unroll
   for (int q = 0; q < RPT; ++q) h[q] = 0.0;
   double oy = 0.0, ov = 0.0;  // wave 0: owner partial sums of lane s
+  if constexpr (PAIR) {
+  // columns in pairs: one barrier and interleaved reductions per pair
+  for (int i0 = 0; i0 < ncol; i0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; u += 2) {
+      const int i = i0 + u;
+      if (i < ncol) {
+        const bool two = i + 1 < ncol;
+        // the columns D-2 and D-1 ahead into the slots freed by i-2, i-1
+        fetch(i + D - 2, ring[(u + D - 2) % D], ryv[(u + D - 2) % D], rvv[(u + D - 2) % D]);
+        fetch(i + D - 1, ring[(u + D - 1) % D], ryv[(u + D - 1) % D], rvv[(u + D - 1) % D]);
+        const double (&v0)[RPT] = ring[u];
+        const double (&v1)[RPT] = ring[(u + 1) % D];
+        // (two partial sums per column: shorter dependent chains)
+        double ga[2] = {0.0, 0.0}, gb[2] = {0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+          const double c = cvl[tid + FG_T * q];
+          ga[q & 1] += v0[q] * c;
+          gb[q & 1] += v1[q] * c;
+        }
+        double g0 = ga[0] + ga[1], g1 = gb[0] + gb[1];
+        // wave 0 forms the panel terms of both columns (lanes s >= t add
+        // zero); every wave then forms r_j itself
+        const int pb = (i >> 1) & 1;
+        if (wave == 0) {
+          const double y0 = lane < t ? ryv[u] : 0.0, w0 = lane < t ? rvv[u] : 0.0;
+          const double y1 = lane < t ? ryv[(u + 1) % D] : 0.0;
+          const double w1 = lane < t ? rvv[(u + 1) % D] : 0.0;
+          double P0 = y0 * dsu + w0 * dsx, Q0 = y0 * ukl + w0 * xkl;
+          double P1 = y1 * dsu + w1 * dsx, Q1 = y1 * ukl + w1 * xkl;
+          wave_total3(g0, P0, Q0);
+          wave_total3(g1, P1, Q1);
+          if (lane == 0) {
+            red[pb][2 * NW + 2] = P0;
+            red[pb][2 * NW + 3] = Q0;
+            red[pb][2 * NW + 4] = P1;
+            red[pb][2 * NW + 5] = Q1;
+          }
+        } else {
+          double z = 0.0;
+          wave_total3(g0, g1, z);
+        }
+        if (lane == 0) {
+          red[pb][wave] = g0;
+          red[pb][NW + wave] = g1;
+        }
+        if (tid == 0) {
+          red[pb][2 * NW] = v0[0];  // row k
+          red[pb][2 * NW + 1] = v1[0];
+        }
+        __syncthreads();
+        double y[2], r[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          double gq[NW / 2];
+#pragma unroll
+          for (int q = 0; q < NW / 2; ++q)
+            gq[q] = red[pb][c * NW + 2 * q] + red[pb][c * NW + 2 * q + 1];
+#pragma unroll
+          for (int h2 = NW / 4; h2 >= 1; h2 /= 2)
+#pragma unroll
+            for (int q = 0; q < h2; ++q) gq[q] += gq[q + h2];
+          const double rk = red[pb][2 * NW + c];
+          const double P = red[pb][2 * NW + 2 + 2 * c], Q = red[pb][2 * NW + 3 + 2 * c];
+          y[c] = 2.0 * rl_quot((gq[0] - P) + snrm * (rk - Q), R.rl, irl);
+          r[c] = rk - Q - ukt * y[c];
+        }
+        if (!two) y[1] = r[1] = 0.0;  // past the group's last column
+        if (wave == 0) {
+          if (lane == 0) {
+            yst[i] = y[0];
+            rst[i] = r[0];
+            if (two) {
+              yst[i + 1] = y[1];
+              rst[i + 1] = r[1];
+            }
+          }
+          // owner slots of lane s: y_s^T r (s < t), y_t^T r (s = t),
+          // |r|^2 (s = 63); v_s^T r (s < t)
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const double ys = c == 0 ? ryv[u] : ryv[(u + 1) % D];
+            const double vs = c == 0 ? rvv[u] : rvv[(u + 1) % D];
+            oy += (lane < t ? ys : (lane == t ? y[c] : (lane == 63 ? r[c] : 0.0))) * r[c];
+            ov += (lane < t ? vs : 0.0) * r[c];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) h[q] += v0[q] * r[0] + v1[q] * r[1];
+      }
+    }
+  }
+  } else {
   for (int i0 = 0; i0 < ncol; i0 += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
@@ -799,25 +944,40 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
         // the column D-1 ahead into the slot freed by column i-1
         fetch(i + D - 1, ring[(u + D - 1) % D], ryv[(u + D - 1) % D], rvv[(u + D - 1) % D]);
         const double (&v)[RPT] = ring[u];
-        double gp = 0.0;
+        // (four partial sums: a shorter dependent chain)
+        double g4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < RPT; ++q) gp += v[q] * cvl[tid + FG_T * q];
-        // every wave forms the panel terms of column j (lanes s >= t add
-        // zero) and then r_j itself: one barrier per column, with the
-        // partial g double-buffered by column parity
+        for (int q = 0; q < RPT; ++q) g4[q & 3] += v[q] * cvl[tid + FG_T * q];
+        double gp = (g4[0] + g4[1]) + (g4[2] + g4[3]);
+        // wave 0 forms the panel terms of column j (lanes s >= t add zero);
+        // every wave then forms r_j itself: one barrier per column, with
+        // the partials double-buffered by column parity
         const double yv = lane < t ? ryv[u] : 0.0, vv = lane < t ? rvv[u] : 0.0;
-        const double P = wave_total(yv * dsu + vv * dsx);
-        const double Q = wave_total(yv * ukl + vv * xkl);
-        gp = wave_total(gp);
         const int pb = i & 1;
+        if (wave == 0) {
+          double P = yv * dsu + vv * dsx, Q = yv * ukl + vv * xkl;
+          wave_total3(gp, P, Q);
+          if (lane == 0) {
+            red[pb][NW + 1] = P;
+            red[pb][NW + 2] = Q;
+          }
+        } else {
+          gp = wave_total(gp);
+        }
         if (lane == 0) red[pb][wave] = gp;
         if (tid == 0) red[pb][NW] = v[0];  // row k
         __syncthreads();
-        double g = 0.0;
+        const double P = red[pb][NW + 1], Q = red[pb][NW + 2];
+        double gq[NW / 2];
 #pragma unroll
-        for (int q = 0; q < NW; ++q) g += red[pb][q];
+        for (int q = 0; q < NW / 2; ++q) gq[q] = red[pb][2 * q] + red[pb][2 * q + 1];
+#pragma unroll
+        for (int h2 = NW / 4; h2 >= 1; h2 /= 2)
+#pragma unroll
+          for (int q = 0; q < h2; ++q) gq[q] += gq[q + h2];
+        const double g = gq[0];
         const double rk = red[pb][NW];
-        const double y = 2.0 * R.dot(g - P, rk - Q);
+        const double y = 2.0 * rl_quot((g - P) + snrm * (rk - Q), R.rl, irl);
         const double r = rk - Q - ukt * y;
         if (wave == 0) {
           if (lane == 0) {
@@ -833,6 +993,7 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
         for (int q = 0; q < RPT; ++q) h[q] += v[q] * r;
       }
     }
+  }
   }
   __syncthreads();  // yst, rst
   // y_t and r on the group's columns
@@ -879,11 +1040,12 @@ static void launch_fused_gemv(const StepArgs<TS>& a, int G, int nldot, bool cloc
     else
       hipLaunchKernelGGL(kern, dim3(G), dim3(FG_T), 0, st, a, nldot);
   };
-  // ring depths: as many columns as the 256 registers hold without spills
-  if (rows > FG_T * 8) go(bd_fused_gemv<TS, 16, 3>);
-  else if (rows > FG_T * 4) go(bd_fused_gemv<TS, 8, 8>);
-  else if (rows > FG_T * 2) go(bd_fused_gemv<TS, 4, 8>);
-  else go(bd_fused_gemv<TS, 2, 8>);
+  // ring depths: as many columns as the 256 registers hold without
+  // spills; columns in pairs (an even depth) where the registers allow
+  if (rows > FG_T * 8) go(bd_fused_gemv<TS, 16, 3, false>);
+  else if (rows > FG_T * 4) go(bd_fused_gemv<TS, 8, 8, true>);
+  else if (rows > FG_T * 2) go(bd_fused_gemv<TS, 4, 8, true>);
+  else go(bd_fused_gemv<TS, 2, 8, true>);
 }
 
 // right_fin: one row in [k+1, m) per lane, FIN rows per group of NWF (4
