@@ -690,7 +690,9 @@ def summarize_trace(trace_dir: str, bench_json: str, plain_json: str, m: int, n:
         line = json.loads(open(plain_json).read().strip().splitlines()[-1])
         rf = line["roofline"]
         kern = rf["kernel"]
-        c, us = agg.get(next(sn for sn in agg if kern in sn), (0, 0.0))
+        # every instantiation of the kernel (template variants by shape)
+        c = sum(v[0] for sn, v in agg.items() if kern in sn)
+        us = sum(v[1] for sn, v in agg.items() if kern in sn)
         out += ["", f"Agreement with the bench line: run without the profiler, the bench "
                 f"reports `roofline.avg_launch_us` = {rf['avg_launch_us']:.2f} us for `{kern}` "
                 f"(start and stop stamps of every launch in its kernel-timed steps, "
