@@ -96,11 +96,14 @@ __global__ void bt_pack(int rows, int nb, const T* S, int lda, T* Wd, int ldw) {
 // T^{-1} = I/2 + strict_upper(G) (nb x nb, ld nb) for the block
 // H_0 ... H_{nb-1} = I - W T W^T of unit-norm reflectors (tau = 2).
 template <typename T>
-__global__ void bt_tinv(int nb, const T* G, T* Ti) {
+__global__ void bt_tinv(int nb, T* G, T* Ti) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= nb * nb) return;
   const int i = e % nb, j = e / nb;
   Ti[e] = i < j ? G[e] : (i == j ? T(0.5) : T(0));
+  // G's strict lower part to zero: G then receives T (upper triangle only)
+  // and is used as a full matrix
+  if (i > j) G[e] = T(0);
 }
 
 // Library GEMM / triangular solve in the precision of the block.
@@ -143,21 +146,27 @@ static int side_stream(SideStream** out) {
 }
 
 void backtrans_plan(ArenaPlan& p, int m, int n) {
-  p.add<double>((size_t)NBB * NBB * 2);  // G, T^{-1}
+  p.add<double>((size_t)NBB * NBB * 2);  // G (then T), T^{-1}
   p.add<double>((size_t)NBB * n);        // Z
   p.add<double>((size_t)m * NBB);        // packed reflectors
+  p.add<double>((size_t)NBB * n);        // T Z
   // the right-hand application's own set
   p.add<double>((size_t)NBB * NBB * 2);
   p.add<double>((size_t)NBB * n);
   p.add<double>((size_t)n * NBB);
+  p.add<double>((size_t)NBB * n);
 }
 
 // Apply H_k0 ... H_k1-1 (blocks, last block first) to the rows of C:
-// Z = W^T C, Z = T Z (triangular solve with T^{-1}), C -= W Z.
+// Z = W^T C, Z = T Z, C -= W Z.  With Z2 (FP64): T is formed once per block
+// from T^{-1} (triangular inverse of the nb x nb matrix) and applied as a
+// GEMM into Z2 (nb x ncol); else by a triangular solve with T^{-1} in place
+// (the solve's blocked small products took about a third of the
+// back-transformation at 8192^2).
 // left: w_k = A[k:m, k];  right: w_k = A[k, k+1:n] acting on rows k+1..n-1.
 template <typename T>
 static int apply_blocks(bool left, int m, int n, const T* A, int lda, int nref, T* C, int ldc,
-                        int ncol, T* G, T* Ti, T* Z, T* Wd, hipStream_t st) {
+                        int ncol, T* G, T* Ti, T* Z, T* Wd, hipStream_t st, T* Z2 = nullptr) {
   const int nblk = (int)ceil_div(nref, NBB);
   for (int bi = nblk - 1; bi >= 0; --bi) {
     const int k0 = bi * NBB;
@@ -178,8 +187,17 @@ static int apply_blocks(bool left, int m, int n, const T* A, int lda, int nref, 
                        G, Ti);
     DDC_LAUNCHED();
     DDC_TRY(gemm_t('T', 'N', nb, ncol, rows, T(1), Wd, rows, Csub, ldc, T(0), Z, nb, st));
-    DDC_TRY(trsm_t(nb, ncol, Ti, nb, Z, nb, st));
-    DDC_TRY(gemm_t('N', 'N', rows, ncol, nb, T(-1), Wd, rows, Z, nb, T(1), Csub, ldc, st));
+    T* Zt = Z;
+    if constexpr (sizeof(T) == 8) {
+      if (Z2) {
+        // T = (T^{-1})^{-1} into G (no longer needed), then Z2 = T Z
+        DDC_TRY(dtrtri_upper(nb, Ti, nb, G, nb, st));
+        DDC_TRY(gemm_t('N', 'N', nb, ncol, nb, T(1), G, nb, Z, nb, T(0), Z2, nb, st));
+        Zt = Z2;
+      }
+    }
+    if (Zt == Z) DDC_TRY(trsm_t(nb, ncol, Ti, nb, Z, nb, st));
+    DDC_TRY(gemm_t('N', 'N', rows, ncol, nb, T(-1), Wd, rows, Zt, nb, T(1), Csub, ldc, st));
   }
   return DDC_OK;
 }
@@ -192,11 +210,13 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
   double* Ti = G ? G + NBB * NBB : nullptr;
   double* Z = ws.take<double>((size_t)NBB * n);
   double* Wd = ws.take<double>((size_t)m * NBB);
+  double* TZ = ws.take<double>((size_t)NBB * n);
   double* G2 = ws.take<double>((size_t)NBB * NBB * 2);
   double* Ti2 = G2 ? G2 + NBB * NBB : nullptr;
   double* Z2 = ws.take<double>((size_t)NBB * n);
   double* Wd2 = ws.take<double>((size_t)n * NBB);
-  if (!Wd2) return DDC_E_NOMEM;
+  double* TZ2 = ws.take<double>((size_t)NBB * n);
+  if (!TZ2) return DDC_E_NOMEM;
   SideStream* sd = nullptr;
   DDC_TRY(side_stream(&sd));
   dim3 tb(32, 8);
@@ -212,12 +232,12 @@ int backtrans_run(int m, int n, const double* A, int lda, const double* Ut,
     DDC_HIP(hipStreamWaitEvent(sd->s, sd->fork, 0));
   }
   // left: reflectors 0..n-1 (the last one is the 1 x 1 sign flip when m == n)
-  const int rcl = apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st);
+  const int rcl = apply_blocks(true, m, n, A, lda, n, U, ldu, n, G, Ti, Z, Wd, st, TZ);
   // right: reflectors 0..n-3 (row n-2 holds no reflection); joined even on
   // failure, so that nothing of this call still runs when it returns
   int rcr = DDC_OK;
   if (right) {
-    rcr = apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G2, Ti2, Z2, Wd2, sd->s);
+    rcr = apply_blocks(false, m, n, A, lda, n - 2, V, ldv, n, G2, Ti2, Z2, Wd2, sd->s, TZ2);
     DDC_HIP(hipEventRecord(sd->join, sd->s));
     DDC_HIP(hipStreamWaitEvent(st, sd->join, 0));
   }

@@ -110,6 +110,16 @@ int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
   return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_dtrsm", s);
 }
 
+int dtrtri_upper(int N, const double* A, int lda, double* inv, int ldinv, hipStream_t st) {
+  if (N <= 0) return DDC_OK;
+  std::lock_guard<std::mutex> lk(g_blas_mu);
+  rocblas_handle h;
+  DDC_TRY(blas_handle(st, &h));
+  rocblas_status s = rocblas_dtrtri(h, rocblas_fill_upper, rocblas_diagonal_non_unit, N, A, lda,
+                                    inv, ldinv);
+  return s == rocblas_status_success ? DDC_OK : blas_fail("rocblas_dtrtri", s);
+}
+
 int dsyrk_upper_t(int N, int K, const double* A, int lda, double* C, int ldc, hipStream_t st) {
   if (N <= 0) return DDC_OK;
   std::lock_guard<std::mutex> lk(g_blas_mu);

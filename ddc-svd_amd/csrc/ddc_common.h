@@ -201,6 +201,8 @@ constexpr int TRAIL_TILE = 128;  // rows per partial of dgemm_trail64
 int dtrsm_left_upper(int M, int N, const double* T, int ldt, double* B, int ldb,
                      hipStream_t st);
 int strsm_left_upper(int M, int N, const float* T, int ldt, float* B, int ldb, hipStream_t st);
+// inv (N x N, upper) = T^{-1} for the N x N upper triangular T (non-unit).
+int dtrtri_upper(int N, const double* A, int lda, double* inv, int ldinv, hipStream_t st);
 // Upper triangle of C (N x N) = A^T A for the K x N matrix A.
 int dsyrk_upper_t(int N, int K, const double* A, int lda, double* C, int ldc, hipStream_t st);
 
