@@ -70,10 +70,13 @@ def check_svd(A, s, U, V, sig_tol=1e-12, vec=True):
 # (2500, 200) and (4200, 130): several 2048-row chunks of the left GEMV and
 # several 512-row blocks of the right GEMV
 # (70, 66): the panel boundary k = 64 is column n-2, whose left GEMV partial
-# sums come from the trailing update and go to the last left finalization
+# sums come from the trailing update and go to the last left finalization.
+# (8250, 120): steps k < 58 (more than 8192 rows) run the two-pass GEMVs
+# (the panel start k = 32 with the trailing update's partial sums), k = 58
+# onward the one-pass GEMV, its first step inside a panel.
 BIDIAG_SHAPES = [(1, 1), (5, 1), (2, 2), (3, 2), (3, 3), (8, 3), (33, 33), (64, 64),
                  (96, 64), (100, 70), (70, 66), (130, 97), (300, 300), (2500, 200),
-                 (4200, 130)]
+                 (4200, 130), (8250, 120)]
 
 
 @pytest.mark.parametrize("shape", BIDIAG_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
@@ -269,9 +272,12 @@ def test_bdsvd_vectors_vs_golden_twisted(ddcsvd):
 
 # ---------------------------------------------------------------- full svd_gpu
 
+# (8193, 300), (300, 8193): step 0 has more than 8192 rows and runs the
+# two-pass GEMVs, the later steps the one-pass GEMV, whose first step takes
+# its panel dots from the two-pass step's finalization
 SVD_SHAPES = [(1, 1), (5, 1), (1, 5), (2, 2), (3, 3), (7, 5), (5, 7), (33, 31), (64, 64),
               (96, 64), (64, 96), (130, 100), (300, 300), (512, 256), (768, 500), (256, 768),
-              (4200, 300), (300, 4200)]
+              (4200, 300), (300, 4200), (8193, 300), (300, 8193)]
 
 
 @pytest.mark.parametrize("shape", SVD_SHAPES, ids=lambda s: f"{s[0]}x{s[1]}")
