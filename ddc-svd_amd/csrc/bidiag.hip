@@ -1002,11 +1002,12 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
     a.Y[j + (int64_t)t * a.ldy] = yst[i];
     a.rvec[j] = rst[i];
   }
-  // the reflector of column k: rows k + w S .. in slices
+  // the reflector of column k: rows k + w S .. in slices, c from LDS
+  // (rows k..m-1 all held: m - k <= FG_T RPT)
   {
     const int S = (m - k + G - 1) / G;
     for (int i = k + w * S + tid; i < min(m, k + (w + 1) * S); i += FG_T) {
-      const double u = R.vec(a.cvec[i], i == k);
+      const double u = R.vec(cvl[i - k], i == k);
       a.A[i + (int64_t)k * a.lda] = (TS)u;
       if (a.Ucopy) a.Ucopy[i + (int64_t)t * a.lda] = u;
     }
@@ -1106,7 +1107,9 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
   }
   const int nrn = noright ? 0 : a.nrn;
-  double c = 0.0, ut = 0.0, rk1 = 0.0, rw = 0.0, w1 = 0.0, rn = 0.0, yk = 0.0, vk = 0.0;
+  constexpr int NRN = 4;  // partial norms per lane in the first batch (256)
+  double c = 0.0, ut = 0.0, rk1 = 0.0, rw = 0.0, w1 = 0.0, yk = 0.0, vk = 0.0;
+  double rn[NRN];
   const int jw = k + 1 + blockIdx.x * FIN + lane;  // this lane's column of the reflector
   if (wave == 0) {
     c = next ? (double)a.A[rc + (int64_t)(k + 1) * a.lda] : 0.0;
@@ -1114,7 +1117,9 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     rk1 = a.rvec[k + 1];
     rw = a.rvec[jw < a.n ? jw : k + 1];
     w1 = (td <= t) ? a.Y[(k + 1) + (int64_t)td * a.ldy] : a.V[(k + 1) + (int64_t)(td - t - 1) * a.ldy];
-    rn = a.rnorm[min(lane, max(nrn - 1, 0))];  // used if lane < nrn
+#pragma unroll
+    for (int u = 0; u < NRN; ++u)  // used if lane + 64 u < nrn
+      rn[u] = a.rnorm[min(lane + FIN * u, max(nrn - 1, 0))];
   } else if (wave == 1) {
     const int tk = min(lane, t);
     yk = a.Y[(k + 1) + (int64_t)tk * a.ldy];
@@ -1151,9 +1156,12 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
   Refl R;
   R.nrm = R.sgn = R.rl = 0.0;
   if (wave == 0) {
-    double r2 = lane < nrn ? rn : 0.0;
-    for (int i = lane + FIN; i < nrn; i += FIN) r2 += a.rnorm[i];
-    if (!noright) R.make(wave_sum(r2), rk1);
+    double r2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < NRN; ++u)
+      if (lane + FIN * u < nrn) r2 += rn[u];
+    for (int i = lane + FIN * NRN; i < nrn; i += FIN) r2 += a.rnorm[i];
+    if (!noright) R.make(wave_total(r2), rk1);
     if (lane == 0) {
       rfl[0] = R.nrm;
       rfl[1] = R.sgn;
@@ -1214,7 +1222,7 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     }
   }
   if (dots) cnl[lane] = cnew;
-  const double s = wave_sum(cc2);
+  const double s = wave_total(cc2);
   if (lane == 0) a.cnorm[blockIdx.x] = s;
   // the right reflector to row k of A and to column t of V (no work-group
   // of this launch reads either: rows > k of A, V[k+1, s < t])
@@ -1252,7 +1260,7 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     if (sx < t) Ld[slot < PW ? sx : t + 1 + sx] = tot;
   }
   if (wave == 0) {
-    const double du = wave_sum(ut * cl), dx = wave_sum(xrow * cl);
+    const double du = wave_total(ut * cl), dx = wave_total(xrow * cl);
     if (lane == 0) {
       Ld[t] = du;
       Ld[2 * t + 1] = dx;
