@@ -769,9 +769,14 @@ __global__ __launch_bounds__(FG_T, 1) void bd_fused_gemv(StepArgs<TS> a, int nld
 #pragma unroll
   for (int u = 0; u < NCN; ++u) cnb[u] = a.cnorm[min(lane + 64 * u, a.ncn - 1)];
   const int dl = lane < 2 * t ? lane : 0;
+  if (lane < 2 * t) {  // the lanes that use them only
 #pragma unroll
-  for (int u = 0; u < NLD; ++u)
-    ldb[u] = a.Ldot[(int64_t)min(wave + NW * u, max(nldot - 1, 0)) * (2 * NB) + dl];
+    for (int u = 0; u < NLD; ++u)
+      ldb[u] = a.Ldot[(int64_t)min(wave + NW * u, max(nldot - 1, 0)) * (2 * NB) + dl];
+  } else {
+#pragma unroll
+    for (int u = 0; u < NLD; ++u) ldb[u] = 0.0;
+  }
   const int sl = min(lane, max(t - 1, 0));
   const int slw = wave == 0 ? sl : 0;
   const double ukr = (double)a.A[k + (int64_t)(a.p + sl) * a.lda];
@@ -1099,13 +1104,19 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
   const bool dotr = !noright && lane < 2 * t + 1;
   const int td = dotr ? lane : 0;
   const int ncl = noright ? 1 : nccR;
+  // (the dot loads on the 2t+1 lanes that use them only: idle quarter-waves
+  // return no data)
   double pd[PB], ph[PB];
+  if (dotr) {
 #pragma unroll
-  for (int u = 0; u < PB; ++u) {
-    const int q = min(wave + NWF * u, ncl - 1);
-    pd[u] = a.Rdot[td + (int64_t)q * DSTRIDE];
-    ph[u] = a.Rpart[rc + (int64_t)q * a.lda];
+    for (int u = 0; u < PB; ++u)
+      pd[u] = a.Rdot[td + (int64_t)min(wave + NWF * u, ncl - 1) * DSTRIDE];
+  } else {
+#pragma unroll
+    for (int u = 0; u < PB; ++u) pd[u] = 0.0;
   }
+#pragma unroll
+  for (int u = 0; u < PB; ++u) ph[u] = a.Rpart[rc + (int64_t)min(wave + NWF * u, ncl - 1) * a.lda];
   const int nrn = noright ? 0 : a.nrn;
   constexpr int NRN = 4;  // partial norms per lane in the first batch (256)
   double c = 0.0, ut = 0.0, rk1 = 0.0, rw = 0.0, w1 = 0.0, yk = 0.0, vk = 0.0;
