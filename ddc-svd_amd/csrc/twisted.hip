@@ -39,8 +39,10 @@
 //    unit vectors);
 //  * Rayleigh-Ritz: lanes of one block whose refined values have a relative
 //    gap in sigma^2 below TW_GTOL form a group.  For each group, the left and
-//    right invariant subspaces are recomputed by two steps of inverse
-//    iteration per member (own shift, pseudo-random start) and orthonormalised
+//    right invariant subspaces are recomputed by three steps of inverse
+//    iteration per member (pseudo-random start; own shift, or for a run of
+//    members closer than TW_TIGHT eps one shift just below the run) and
+//    orthonormalised
 //    (Gram-Schmidt, twice); the k x k matrix M = U_c^T B V_c is diagonalised
 //    by one-sided Jacobi and the bases are rotated by its singular vectors.
 #include <algorithm>
@@ -55,6 +57,8 @@ constexpr double TW_TINY = 2.2250738585072014e-308;
 constexpr double TW_SPLIT = 16.0;  // split |b_i| <= TW_SPLIT eps sigma_max
 constexpr double TW_GTOL = 1e-6;   // relative gap (in sigma^2) that forms a group
 constexpr int TW_KMAX = 64;        // members per Rayleigh-Ritz window
+constexpr double TW_TIGHT = 1024.0;  // members closer than this (eps, relative) share a shift
+constexpr int TW_RRIT = 3;          // inverse-iteration steps per Rayleigh-Ritz member
 constexpr int TW_CK = 64;          // checkpoint interval = LDS chunk
 constexpr int TW_MAXBIS = 200;     // bisection steps per lane at most
 constexpr double TW_CONV = 32.0;   // Rayleigh correction accepted below TW_CONV eps lam
@@ -757,21 +761,39 @@ __global__ __launch_bounds__(256) void tw_rr(RrArgs A) {
   const double smax = A.smax;
   const double pivmin = TW_TINY * fmax(1.0, smax * smax);
 
-  // shifts: the members' values, kept 4 eps apart so that equal values do
-  // not share a factorization
+  // shifts (in sigma^2).  A member whose neighbours in the group lie more
+  // than TW_TIGHT eps away takes its own value.  Members closer than that
+  // cannot be told apart by their own shifts (the values are accurate to a
+  // few eps): a shift within ulps of a run of such values amplifies
+  // whichever vector of the run lies nearest, every member's iterate
+  // collapses onto it, and Gram-Schmidt of nearly parallel vectors loses the
+  // run's other directions.  The members of such a run share one shift
+  // below the run, by twice its width plus 256 eps (and twice the floor,
+  // for runs of zero values), so that every vector of the run is amplified
+  // alike and each member's random start keeps its own mixture of them; the
+  // shift stays nearer the run than half the gap to the member below.
   if (tid == 0) {
-    double prevt = -INFINITY;
+    const double floor2 = TW_EPS * TW_EPS * smax * smax;
     for (int q = 0; q < k; ++q) {
       const double s = A.sigma[mem[q]];
-      double t = fmax(s * s, TW_EPS * TW_EPS * smax * smax);
-      t = fmax(t, prevt + 4.0 * TW_EPS * t);
-      tau_s[q] = t;
-      prevt = t;
+      wtmp[q] = fmax(s * s, floor2);
+    }
+    for (int q = 0; q < k;) {
+      int e = q;
+      while (e + 1 < k && wtmp[e + 1] - wtmp[e] <= TW_TIGHT * TW_EPS * wtmp[e + 1]) ++e;
+      double sh = wtmp[q];
+      if (e > q) {
+        double dl = 2.0 * (wtmp[e] - wtmp[q]) + 256.0 * TW_EPS * wtmp[e] + 2.0 * floor2;
+        if (q > 0) dl = fmin(dl, 0.5 * (wtmp[q] - wtmp[q - 1]));
+        sh = wtmp[q] - dl;
+      }
+      for (int r = q; r <= e; ++r) tau_s[r] = sh;
+      q = e + 1;
     }
   }
   __syncthreads();
 
-  // ---- phase A: two steps of inverse iteration per member ----
+  // ---- phase A: TW_RRIT steps of inverse iteration per member ----
   // waves 0 and 1 (rep 0 and rep 1) compute; waves 2 and 3 only keep the
   // barriers company
   {
@@ -790,7 +812,7 @@ __global__ __launch_bounds__(256) void tw_rr(RrArgs A) {
     const double pm = fmax(pivmin, TW_EPS * tau);
     const int nc = (L + TW_CK - 1) / TW_CK;
     double scale = 1.0;
-    for (int itr = 0; itr < 2; ++itr) {
+    for (int itr = 0; itr < TW_RRIT; ++itr) {
       // forward: pivots of L D L^T - tau I = L+ D+ L+^T, w = L+^{-1} x,
       // y' = w / D+ stored in place
       double s = -tau, wprev = 0.0, lprev = 0.0;

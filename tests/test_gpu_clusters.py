@@ -11,6 +11,8 @@ and assert, besides the values:
   orthogonality   ||U^T U - I||_F, ||V^T V - I||_F <= 2e-13 * mn
   residual        ||A - U S V^T||_F / ||A||_F <= 1e-13 (zero matrix: exact)
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -150,3 +152,23 @@ def test_bdsvd_vectors_tight_chain(ddcsvd):
     gpu_case("bdsvd_vectors tight chain 40")
     s, X, Y = ddcsvd.bdsvd_vectors(a, b)
     check_exact(B, s, Y, X, np.linalg.svd(B, compute_uv=False))
+
+
+def test_bdsvd_vectors_pairs_a_few_ulps_apart(ddcsvd):
+    """Regression input: the bidiagonal of the x2-repeat 1024 matrix of
+    test_repeated_values, as a bidiagonalization with another (equally
+    valid) summation order left it (tests/golden/repeat2_1024_bidiag.npz).
+    Its 512 pairs are unreduced; the pair at sigma = 472 lies 5.3 eps apart
+    in sigma^2.  Shifts within ulps of such a pair made both inverse
+    iterations collapse onto one vector (triplet residual 4e-9 against
+    4e-11 elsewhere); the pair must come out as accurately as the rest."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "repeat2_1024_bidiag.npz"))
+    a, b = g["alpha"], g["beta"]
+    B = np.diag(a) + np.diag(b, 1)
+    exact = np.linalg.svd(B, compute_uv=False)
+    assert np.max(np.abs(np.sort(exact) - np.repeat(np.arange(1.0, 513.0), 2))) <= 1e-11
+    gpu_case("bdsvd_vectors x2 repeats 1024 (fixture)")
+    s, X, Y = ddcsvd.bdsvd_vectors(a, b)
+    check_exact(B, s, Y, X, exact)
+    tri = np.linalg.norm(B @ X - Y * s, axis=0)
+    assert np.max(tri) <= 1e-12 * np.max(exact), f"worst triplet residual {np.max(tri):.3e}"
