@@ -1172,7 +1172,7 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     for (int u = 0; u < NRN; ++u)
       if (lane + FIN * u < nrn) r2 += rn[u];
     for (int i = lane + FIN * NRN; i < nrn; i += FIN) r2 += a.rnorm[i];
-    if (!noright) R.make(wave_total(r2), rk1);
+    if (!noright) R.make(wave_sum(r2), rk1);
     if (lane == 0) {
       rfl[0] = R.nrm;
       rfl[1] = R.sgn;
@@ -1233,7 +1233,7 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     }
   }
   if (dots) cnl[lane] = cnew;
-  const double s = wave_total(cc2);
+  const double s = wave_sum(cc2);
   if (lane == 0) a.cnorm[blockIdx.x] = s;
   // the right reflector to row k of A and to column t of V (no work-group
   // of this launch reads either: rows > k of A, V[k+1, s < t])
@@ -1271,7 +1271,7 @@ __global__ __launch_bounds__(NWF * FIN) void bd_right_fin(StepArgs<TS> a, int nc
     if (sx < t) Ld[slot < PW ? sx : t + 1 + sx] = tot;
   }
   if (wave == 0) {
-    const double du = wave_total(ut * cl), dx = wave_total(xrow * cl);
+    const double du = wave_sum(ut * cl), dx = wave_sum(xrow * cl);
     if (lane == 0) {
       Ld[t] = du;
       Ld[2 * t + 1] = dx;
