@@ -48,7 +48,6 @@ import ctypes
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
 
