@@ -45,6 +45,12 @@
 //    orthonormalised
 //    (Gram-Schmidt, twice); the k x k matrix M = U_c^T B V_c is diagonalised
 //    by one-sided Jacobi and the bases are rotated by its singular vectors.
+//    Windows hold at most 64 members; a wider group runs window by window
+//    (one launch each), every window orthogonalised against the group's
+//    earlier windows and, unless its values are at the rounding level of
+//    zero, with its left basis formed as B V_c (independent left and right
+//    iterates of an inseparable run wider than a window span different
+//    subspaces of it).
 #include <algorithm>
 #include <vector>
 
@@ -704,7 +710,10 @@ struct RrArgs {
   const double* a;        // the bidiagonal (split as in qd)
   const double* b;
   double tau_split;
-  const int4* groups;     // {member offset, k, block start, block end}
+  const int4* groups;     // per window: {member offset, k, block start, block end}
+  const int4* wins;       // per window: {offset of its group's first member,
+                          //  members of the group before it, group size, 0}
+  int base;               // first window of this launch
   const int* members;     // global indices, ascending within a group
   double* sigma;          // refined values (in/out)
   double* Vt;
@@ -752,14 +761,20 @@ __global__ __launch_bounds__(256) void tw_rr(RrArgs A) {
   __shared__ double part[4][64];
   __shared__ double wtmp[TW_KMAX];
   __shared__ int flag;
-  const int4 g = A.groups[blockIdx.x];
+  const int4 g = A.groups[A.base + blockIdx.x];
+  const int4 gw = A.wins[A.base + blockIdx.x];
   const int moff = g.x, k = g.y, r0 = g.z, r1 = g.w;
+  const int* pmem = A.members + gw.x;  // the group's earlier windows (final)
+  const int np = gw.y;
   const int n = A.n, L = r1 - r0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t ldv = A.ldv;
   const int* mem = A.members + moff;
   const double smax = A.smax;
   const double pivmin = TW_TINY * fmax(1.0, smax * smax);
+  // a group wider than one window takes its left basis from the right one
+  // (phase B), unless its values are at the rounding level of zero
+  const bool fromv = gw.z > TW_KMAX && A.sigma[mem[0]] > 1024.0 * TW_EPS * smax;
 
   // shifts (in sigma^2).  A member whose neighbours in the group lie more
   // than TW_TIGHT eps away takes its own value.  Members closer than that
@@ -878,30 +893,56 @@ __global__ __launch_bounds__(256) void tw_rr(RrArgs A) {
   __syncthreads();
 
   // ---- phase B: Gram-Schmidt, twice, on both bases ----
-  // v_q -= sum_{p<q} (v_p . v_q) v_p, then normalise; rows of the block are
-  // spread over the threads
+  // v_q -= sum_p (v_p . v_q) v_p over the members of the group's earlier
+  // windows (final: earlier launches) and then the window's earlier members,
+  // 64 at a time, then normalise; rows of the block are spread over the
+  // threads.  A group wider than one window (fromv) takes its left basis
+  // from the right one, U_c = B V_c before the orthonormalisation: in a run
+  // of values that no shift separates and that is wider than the window,
+  // independent left and right iterates span different subspaces of the
+  // run, and U_c^T B V_c would not hold its values.
+  auto member = [&](int p) { return p < np ? pmem[p] : mem[p - np]; };
   for (int rep = 0; rep < 2; ++rep) {
     double* M = rep ? A.Ut : A.Vt;
+    if (rep == 1 && fromv) {
+      for (int e = wave; e < L; e += 4) {
+        const int i = r0 + e;
+        const int64_t row = (int64_t)i * ldv;
+        if (lane < k) {
+          double bv = A.a[i] * A.Vt[row + mem[lane]];
+          if (i + 1 < r1) {
+            const double bi = A.b[i];
+            bv += (fabs(bi) <= A.tau_split ? 0.0 : bi) * A.Vt[row + ldv + mem[lane]];
+          }
+          A.Ut[row + mem[lane]] = bv;
+        }
+      }
+      __syncthreads();
+    }
     for (int q = 0; q < k; ++q) {
       for (int pass = 0; pass < 2; ++pass) {
-        // dots with the earlier members: lane p of every wave, rows split
-        // over the four waves
-        if (q > 0) {
+        // dots with earlier members: lane p of every wave, rows split over
+        // the four waves
+        for (int p0 = 0; p0 < np + q; p0 += 64) {
+          const int cnt = min(64, np + q - p0);
           double acc = 0.0;
-          if (lane < q)
+          if (lane < cnt) {
+            const int mp = member(p0 + lane);
             for (int e = wave; e < L; e += 4) {
               const int64_t row = (int64_t)(r0 + e) * ldv;
-              acc += M[row + mem[lane]] * M[row + mem[q]];
+              acc += M[row + mp] * M[row + mem[q]];
             }
+          }
           __syncthreads();
           buf[0][wave][lane] = acc;
           __syncthreads();
-          if (tid < q) coef[tid] = buf[0][0][tid] + buf[0][1][tid] + buf[0][2][tid] + buf[0][3][tid];
+          if (tid < cnt)
+            coef[tid] = buf[0][0][tid] + buf[0][1][tid] + buf[0][2][tid] + buf[0][3][tid];
           __syncthreads();
           for (int e = tid; e < L; e += 256) {
             const int64_t row = (int64_t)(r0 + e) * ldv;
             double v = M[row + mem[q]];
-            for (int p = 0; p < q; ++p) v -= coef[p] * M[row + mem[p]];
+            for (int p = 0; p < cnt; ++p) v -= coef[p] * M[row + member(p0 + p)];
             M[row + mem[q]] = v;
           }
           __syncthreads();
@@ -1172,7 +1213,8 @@ void twisted_plan(ArenaPlan& p, int n) {
   p.add<double>(n);                    // D&C values
   p.add<double>(2 * (size_t)n);        // brackets
   p.add<int>(9 * (size_t)n + 64);      // split, bs, be, cs, ce, r0, r1, flags, blk, count
-  p.add<int>(4 * (size_t)n + 4);       // group table (int4 per group, at most n / 2 + n / 64)
+  p.add<int>(4 * (size_t)n + 4);       // window table (int4 per window, at most n / 2 + n / 64)
+  p.add<int>(4 * (size_t)n + 4);       // window's group (int4 per window)
   p.add<int>(n);                       // group members
 }
 
@@ -1185,6 +1227,7 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   double* lam = ws.take<double>(2 * (size_t)n);
   int* ib = ws.take<int>(9 * (size_t)n + 64);
   int4* gtab = ws.take<int4>((size_t)n + 1);
+  int4* gwin = ws.take<int4>((size_t)n + 1);
   int* gmem = ws.take<int>(n);
   if (!gmem) return DDC_E_NOMEM;
   int* split = ib;
@@ -1274,17 +1317,22 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   // per block: its last lane, the first lane and size of its open group;
   // members of a group are chained through nxt
   std::vector<int> last(n, -1), gst(n, -1), gsz(n, 0), nxt(n, -1);
-  std::vector<int4> tab;
-  std::vector<int> mem, cur;
+  // A group is cut into windows of TW_KMAX members (the last may hold
+  // one); window w of a group runs in launch w, after the group's earlier
+  // windows, against whose final vectors it is orthogonalised.
+  std::vector<int4> tab, win;
+  std::vector<int> mem, cur, lvl;
   auto close = [&](int kb) {
     if (gsz[kb] < 2) return;
     cur.clear();
     for (int j = gst[kb]; j >= 0; j = nxt[j]) cur.push_back(j);
-    for (size_t w = 0; w + 1 < cur.size(); w += TW_KMAX) {
+    const int g0 = (int)mem.size();
+    mem.insert(mem.end(), cur.begin(), cur.end());
+    for (size_t w = 0; w < cur.size(); w += TW_KMAX) {
       const int k = (int)std::min<size_t>(TW_KMAX, cur.size() - w);
-      if (k < 2) break;
-      tab.push_back(make_int4((int)mem.size(), k, kb, hbe[kb]));
-      mem.insert(mem.end(), cur.begin() + w, cur.begin() + w + k);
+      tab.push_back(make_int4(g0 + (int)w, k, kb, hbe[kb]));
+      win.push_back(make_int4(g0, (int)w, (int)cur.size(), 0));
+      lvl.push_back((int)(w / TW_KMAX));
     }
   };
   for (int j = 0; j < n; ++j) {
@@ -1304,7 +1352,22 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
   for (int kb = 0; kb < n; ++kb)
     if (last[kb] >= 0) close(kb);
   if (!tab.empty()) {
-    DDC_HIP(hipMemcpyAsync(gtab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice, st));
+    // windows in launch order: every group's first window, then the second ...
+    int nlev = 0;
+    for (int l : lvl) nlev = std::max(nlev, l + 1);
+    std::vector<int4> tabl, winl;
+    std::vector<int> lstart(nlev + 1, 0);
+    for (int l = 0; l < nlev; ++l) {
+      lstart[l] = (int)tabl.size();
+      for (size_t i = 0; i < tab.size(); ++i)
+        if (lvl[i] == l) {
+          tabl.push_back(tab[i]);
+          winl.push_back(win[i]);
+        }
+    }
+    lstart[nlev] = (int)tabl.size();
+    DDC_HIP(hipMemcpyAsync(gtab, tabl.data(), sizeof(int4) * tabl.size(), hipMemcpyHostToDevice, st));
+    DDC_HIP(hipMemcpyAsync(gwin, winl.data(), sizeof(int4) * winl.size(), hipMemcpyHostToDevice, st));
     DDC_HIP(hipMemcpyAsync(gmem, mem.data(), sizeof(int) * mem.size(), hipMemcpyHostToDevice, st));
     RrArgs ra;
     ra.n = n;
@@ -1313,6 +1376,7 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
     ra.b = b;
     ra.tau_split = tau_split;
     ra.groups = gtab;
+    ra.wins = gwin;
     ra.members = gmem;
     ra.sigma = sigma;
     ra.Vt = Vt;
@@ -1321,8 +1385,11 @@ int twisted_run(int n, const double* a, const double* b, double* sigma,
     ra.ck = scr;
     ra.nck = (n + TW_CK - 1) / TW_CK;
     ra.smax = smax;
-    hipLaunchKernelGGL(tw_rr, dim3((unsigned)tab.size()), dim3(256), 0, st, ra);
-    DDC_LAUNCHED();
+    for (int l = 0; l < nlev; ++l) {
+      ra.base = lstart[l];
+      hipLaunchKernelGGL(tw_rr, dim3((unsigned)(lstart[l + 1] - lstart[l])), dim3(256), 0, st, ra);
+      DDC_LAUNCHED();
+    }
     // the host vectors must outlive the copies
     DDC_HIP(hipStreamSynchronize(st));
   }

@@ -82,7 +82,7 @@ def test_haar_orthogonal(ddcsvd, n):
     check_exact(A, s, U, V, np.ones(n))
 
 
-@pytest.mark.parametrize("mult,n", [(3, 300), (5, 200), (2, 1024)])
+@pytest.mark.parametrize("mult,n", [(3, 300), (5, 200), (2, 1024), (100, 200)])
 def test_repeated_values(ddcsvd, mult, n):
     """Q diag(1,..,1, 2,..,2, ...) P^T: every value mult times.  The
     bidiagonal of such a matrix is not split; its values come in clusters of
@@ -172,3 +172,17 @@ def test_bdsvd_vectors_pairs_a_few_ulps_apart(ddcsvd):
     check_exact(B, s, Y, X, exact)
     tri = np.linalg.norm(B @ X - Y * s, axis=0)
     assert np.max(tri) <= 1e-12 * np.max(exact), f"worst triplet residual {np.max(tri):.3e}"
+
+
+@pytest.mark.parametrize("n", [100, 200])
+def test_bdsvd_vectors_inseparable_run_beyond_one_window(ddcsvd, n):
+    """Unreduced bidiagonal (a = 1, b = 1e-14, above the split threshold)
+    whose n values all lie within a few eps: one group of n members, more
+    than one Rayleigh-Ritz window of 64, and no value separable from the
+    others.  The windows' vectors must be orthogonal to each other too."""
+    a = np.ones(n)
+    b = np.full(n - 1, 1e-14)
+    B = np.diag(a) + np.diag(b, 1)
+    gpu_case(f"bdsvd_vectors inseparable run of {n}")
+    s, X, Y = ddcsvd.bdsvd_vectors(a, b)
+    check_exact(B, s, Y, X, np.linalg.svd(B, compute_uv=False))
