@@ -456,6 +456,15 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
   const double dn1 = d[n - 1];
 
   double lam = fmin(fmax(sig * sig, lo), hi);
+  // A value whose square is within the pivot guard (0 in particular) is
+  // solved at a shift just below zero (eps^2 smax^2, at least 4 pivmin so
+  // that it is not itself guarded), where the shifted Gram
+  // matrix is positive definite: at a shift of exactly 0 a zero diagonal
+  // entry makes the transforms' quotients 0/0 (taken as 0 by the guard
+  // instead of their limit), every twist index ties and the vector is
+  // wrong.  Its first vector is kept.  (Larger tiny values keep their own
+  // shift: in a graded block they are resolved to high relative accuracy.)
+  if (lam <= pivmin) lam = -fmax(TW_EPS * TW_EPS * smax * smax, 4.0 * pivmin);
   double nz2 = 1.0, corr = 0.0, prev = INFINITY;
   int rfin = g1 - 1;
   bool active = live;
@@ -660,13 +669,14 @@ __global__ __launch_bounds__(128) void tw_solve(TwArgs a) {
     }
     if (!active) continue;
     corr = gbest / nz2;
-    // converged: correction within a few ulps of lam (the shift's error is
+    // converged: a value at the zero floor (lam < 0, its first vector), a
+    // correction within a few ulps of lam (the shift's error is
     // then far below any gap the vector can resolve, and the value returned
     // includes the correction), the vector's norm saturated, or (from the
     // third step on) the correction no longer halving -- it has reached the
     // rounding floor of gamma_r, which for small singular values sits above
     // eps*lam
-    if (fabs(corr) <= TW_CONV * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS) ||
+    if (lam < 0.0 || fabs(corr) <= TW_CONV * TW_EPS * lam || nz2 > 1.0 / (TW_EPS * TW_EPS) ||
         (it >= 2 && fabs(corr) > 0.5 * prev)) {
       active = false;
       continue;

@@ -186,3 +186,29 @@ def test_bdsvd_vectors_inseparable_run_beyond_one_window(ddcsvd, n):
     gpu_case(f"bdsvd_vectors inseparable run of {n}")
     s, X, Y = ddcsvd.bdsvd_vectors(a, b)
     check_exact(B, s, Y, X, np.linalg.svd(B, compute_uv=False))
+
+
+@pytest.mark.parametrize("n", [3, 20, 100])
+def test_bdsvd_vectors_zero_diagonal(ddcsvd, n):
+    """B = the shift matrix (a = 0, b = 1), one unreduced block: values 0
+    and 1 (n - 1 times).  The zero value's vectors are e_0 (right) and
+    e_{n-1} (left).  At a shift of exactly 0 the transforms' quotients are
+    0/0 and every twist index ties; the solve must still find them."""
+    a = np.zeros(n)
+    b = np.ones(n - 1)
+    B = np.diag(a) + np.diag(b, 1)
+    gpu_case(f"bdsvd_vectors zero diagonal {n}")
+    s, X, Y = ddcsvd.bdsvd_vectors(a, b)
+    check_exact(B, s, Y, X, np.r_[0.0, np.ones(n - 1)])
+    assert abs(abs(X[0, 0]) - 1.0) <= 1e-14, "right null vector is not e_0"
+    assert abs(abs(Y[n - 1, 0]) - 1.0) <= 1e-14, "left null vector is not e_(n-1)"
+
+
+def test_bdsvd_vectors_graded_over_200_decades(ddcsvd):
+    """a_i = 10^(-4i), b_i = 10^(-4i-1): values over 200 decades."""
+    a = 10.0 ** (-4.0 * np.arange(50))
+    b = 10.0 ** (-4.0 * np.arange(49) - 1.0)
+    B = np.diag(a) + np.diag(b, 1)
+    gpu_case("bdsvd_vectors graded over 200 decades")
+    s, X, Y = ddcsvd.bdsvd_vectors(a, b)
+    check_exact(B, s, Y, X, np.linalg.svd(B, compute_uv=False))
