@@ -308,14 +308,35 @@ static int copy_d2h(void* dst, size_t dpitch, const void* src, size_t spitch, si
   return DDC_OK;
 }
 
-// flag[0] = 1 if any entry of the m x n matrix is not finite
-__global__ void k_check_finite(int m, int n, const double* A, int lda, int* flag) {
+// flag[0] = 1 if any entry of the m x n matrix is not finite; amax = the
+// bits of max |A| (non-negative doubles order as their bit patterns), one
+// atomic per wave
+__global__ void k_check_finite(int m, int n, const double* A, int lda, int* flag,
+                               unsigned long long* amax) {
   int64_t total = (int64_t)m * n;
+  double mx = 0.0;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t j = e / m, i = e - j * m;
     double v = A[i + j * lda];
     if (!isfinite(v)) flag[0] = 1;
+    else mx = fmax(mx, fabs(v));
+  }
+  if (amax) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, __double_as_longlong(mx));
+  }
+}
+
+// A[i, j] = A[i, j] s1 s2 on an m x n matrix (two powers of two: a scale
+// of 2^1029 for a matrix of subnormal entries is not itself a double)
+__global__ void k_scale(int m, int n, double* A, int lda, double s1, double s2) {
+  int64_t total = (int64_t)m * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t j = e / m, i = e - j * m;
+    A[i + j * lda] = A[i + j * lda] * s1 * s2;
   }
 }
 
@@ -403,15 +424,34 @@ struct Timer {
 // Results: sigma in b.sigma, U (m x n) in dU/ldu, V (n x n) in dV/ldv.
 static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu,
                        double* dV, int ldv, hipStream_t st, Timer& tm) {
-  DDC_HIP(hipMemsetAsync(b.flag, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p, b.flag);
+  // flag[0]: not finite; flag[2..3]: the bits of max |A|
+  unsigned long long* amax = reinterpret_cast<unsigned long long*>(b.flag + 2);
+  DDC_HIP(hipMemsetAsync(b.flag, 0, 4 * sizeof(int), st));
+  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p, b.flag,
+                     amax);
   DDC_LAUNCHED();
-  int hflag = 0;
-  DDC_HIP(hipMemcpyAsync(&hflag, b.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  int hflag[4] = {0, 0, 0, 0};
+  DDC_HIP(hipMemcpyAsync(hflag, b.flag, sizeof(hflag), hipMemcpyDeviceToHost, st));
   DDC_HIP(hipStreamSynchronize(st));
-  if (hflag) {
+  if (hflag[0]) {
     set_err("input matrix holds NaN or Inf");
     return DDC_E_NUMERIC;
+  }
+  // A matrix whose largest entry lies outside [2^-450, 2^450] is scaled by
+  // a power of two (exact) to bring it into [1/2, 1): the values and
+  // vectors stages work with sigma^2, which would underflow or overflow.
+  // sigma is scaled back at the end; U and V do not change.
+  unsigned long long abits = 0;
+  std::memcpy(&abits, hflag + 2, sizeof(abits));
+  double amx = 0.0;
+  std::memcpy(&amx, &abits, sizeof(amx));
+  int aexp = 0;
+  if (amx > 0.0) (void)std::frexp(amx, &aexp);
+  const bool scaled = amx > 0.0 && (aexp < -450 || aexp > 450);
+  if (scaled) {
+    hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p,
+                       std::ldexp(1.0, -aexp / 2), std::ldexp(1.0, -aexp - (-aexp / 2)));
+    DDC_LAUNCHED();
   }
   DDC_TRY(tm.mark());
   DDC_TRY(bidiag_run(m, n, b.Ap, b.lda_p, b.alpha, b.beta, ws, st));
@@ -421,6 +461,11 @@ static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu
   DDC_TRY(twisted_run(n, b.alpha, b.beta, b.sigma, b.Vt, b.Ut, b.ldb, ws, st));
   DDC_TRY(tm.mark());
   DDC_TRY(backtrans_run(m, n, b.Ap, b.lda_p, b.Ut, b.Vt, b.ldb, dU, ldu, dV, ldv, ws, st));
+  if (scaled) {
+    hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, n, 1, b.sigma, n,
+                       std::ldexp(1.0, aexp / 2), std::ldexp(1.0, aexp - aexp / 2));
+    DDC_LAUNCHED();
+  }
   DDC_TRY(tm.mark());
   return DDC_OK;
 }
@@ -778,7 +823,8 @@ static int mixed_dev(int n, const double* dA, int lda, double* dsigma, double* d
   int* flag = ws.take<int>(64);
   if (!flag) return DDC_E_NOMEM;
   DDC_HIP(hipMemsetAsync(flag, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, n, n, dA, lda, flag);
+  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, n, n, dA, lda, flag,
+                     (unsigned long long*)nullptr);
   DDC_LAUNCHED();
   int hflag = 0;
   DDC_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st));

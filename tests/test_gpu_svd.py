@@ -390,6 +390,18 @@ def test_svd_gpu_zero_and_rank_deficient(ddcsvd):
     assert np.linalg.norm(V.T @ V - np.eye(30)) <= 2e-13 * 30
 
 
+@pytest.mark.parametrize("scale", [1e-300, 1e-160, 1e160, 1e300])
+def test_svd_gpu_extreme_scales(ddcsvd, scale):
+    """Entries whose squares under- or overflow: the library scales A by a
+    power of two (exact) and sigma back.  Checked on the matrix brought back
+    to unit scale (the checks' own squares would under- or overflow)."""
+    A = scale * gen.normal_matrix(120, 100, 7)
+    gpu_case(f"svd_gpu 120x100 scaled by {scale:g}")
+    s, U, V, _ = ddcsvd.svd(A)
+    assert np.all(np.isfinite(s)) and np.all(np.isfinite(U)) and np.all(np.isfinite(V))
+    check_svd(A / scale, s / scale, U, V)
+
+
 def test_svd_gpu_rejects_nonfinite(ddcsvd):
     A = np.ones((4, 3))
     A[1, 1] = np.nan
