@@ -709,10 +709,33 @@ static int bdsvd_host(int n, const double* alpha, const double* beta, double* si
   double* da = ws.take<double>(n);
   double* db = ws.take<double>(n);
   double* ds = ws.take<double>(n);
-  DDC_HIP(hipMemcpyAsync(da, alpha, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  // a bidiagonal whose largest entry lies outside [2^-450, 2^450] is scaled
+  // by a power of two (exact, in two factors) and sigma scaled back, as in
+  // the SVD: the values and vectors stages work with sigma^2
+  double amx = 0.0;
+  for (int i = 0; i < n; ++i) amx = std::fmax(amx, std::fabs(alpha[i]));
+  for (int i = 0; i + 1 < n; ++i) amx = std::fmax(amx, std::fabs(beta[i]));
+  int aexp = 0;
+  const bool scaled = amx > 0.0 && std::isfinite(amx) && (std::frexp(amx, &aexp), true) &&
+                      (aexp < -450 || aexp > 450);
+  std::vector<double> sa, sb;
+  const double* pa = alpha;
+  const double* pb = beta;
+  if (scaled) {
+    const double f1 = std::ldexp(1.0, -aexp / 2), f2 = std::ldexp(1.0, -aexp - (-aexp / 2));
+    sa.resize(n);
+    for (int i = 0; i < n; ++i) sa[i] = alpha[i] * f1 * f2;
+    pa = sa.data();
+    if (n > 1) {
+      sb.resize(n - 1);
+      for (int i = 0; i + 1 < n; ++i) sb[i] = beta[i] * f1 * f2;
+      pb = sb.data();
+    }
+  }
+  DDC_HIP(hipMemcpyAsync(da, pa, sizeof(double) * n, hipMemcpyHostToDevice, st));
   DDC_HIP(hipMemsetAsync(db, 0, sizeof(double) * n, st));
   if (n > 1)
-    DDC_HIP(hipMemcpyAsync(db, beta, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
+    DDC_HIP(hipMemcpyAsync(db, pb, sizeof(double) * (n - 1), hipMemcpyHostToDevice, st));
   DDC_TRY(dc_run(n, da, db, ds, ws, st));
   if (vec) {
     double* Vt = ws.take<double>((size_t)ldb * n * 2);
@@ -729,6 +752,10 @@ static int bdsvd_host(int n, const double* alpha, const double* beta, double* si
   }
   DDC_HIP(hipMemcpyAsync(sigma, ds, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   DDC_HIP(hipStreamSynchronize(st));
+  if (scaled) {
+    const double g1 = std::ldexp(1.0, aexp / 2), g2 = std::ldexp(1.0, aexp - aexp / 2);
+    for (int i = 0; i < n; ++i) sigma[i] = sigma[i] * g1 * g2;
+  }
   return DDC_OK;
 }
 

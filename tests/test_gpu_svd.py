@@ -402,6 +402,23 @@ def test_svd_gpu_extreme_scales(ddcsvd, scale):
     check_svd(A / scale, s / scale, U, V)
 
 
+@pytest.mark.parametrize("scale", [1e-300, 1e-160, 1e160, 1e300])
+def test_bdsvd_extreme_scales(ddcsvd, scale):
+    """A bidiagonal at a scale whose squares under- or overflow: values and
+    vectors as for the same bidiagonal at unit scale."""
+    rng = np.random.default_rng(11)
+    a0, b0 = rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 63)
+    a, b = scale * a0, scale * b0
+    B0 = np.diag(a / scale) + np.diag(b / scale, 1)
+    gpu_case(f"bdsvd 64 scaled by {scale:g}")
+    sv = ddcsvd.bdsvd_values(a, b)
+    s, X, Y = ddcsvd.bdsvd_vectors(a, b)
+    ex = np.linalg.svd(B0, compute_uv=False)
+    assert np.all(np.isfinite(sv)) and np.all(np.isfinite(s))
+    assert np.max(np.abs(np.sort(sv) / scale - np.sort(ex))) <= 1e-12 * ex.max()
+    check_svd(B0, s / scale, Y, X)
+
+
 def test_svd_gpu_rejects_nonfinite(ddcsvd):
     A = np.ones((4, 3))
     A[1, 1] = np.nan
