@@ -340,6 +340,45 @@ __global__ void k_scale(int m, int n, double* A, int lda, double s1, double s2) 
   }
 }
 
+// Checks the m x n device matrix A for NaN and Inf (*nonfinite) and, when
+// its largest entry lies outside [2^-450, 2^450], scales it in place by a
+// power of two (exact) into [1/2, 1): *aexp is then the binary exponent
+// that undoes it (x 2^aexp), else 0.  flag: 4 ints of device scratch.
+static int check_and_scale(int m, int n, double* A, int lda, int* flag, hipStream_t st,
+                           bool* nonfinite, int* aexp) {
+  unsigned long long* amax = reinterpret_cast<unsigned long long*>(flag + 2);
+  DDC_HIP(hipMemsetAsync(flag, 0, 4 * sizeof(int), st));
+  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, m, n, A, lda, flag, amax);
+  DDC_LAUNCHED();
+  int hflag[4] = {0, 0, 0, 0};
+  DDC_HIP(hipMemcpyAsync(hflag, flag, sizeof(hflag), hipMemcpyDeviceToHost, st));
+  DDC_HIP(hipStreamSynchronize(st));
+  *nonfinite = hflag[0] != 0;
+  *aexp = 0;
+  unsigned long long abits = 0;
+  std::memcpy(&abits, hflag + 2, sizeof(abits));
+  double amx = 0.0;
+  std::memcpy(&amx, &abits, sizeof(amx));
+  int e = 0;
+  if (amx > 0.0) (void)std::frexp(amx, &e);
+  if (!*nonfinite && amx > 0.0 && (e < -450 || e > 450)) {
+    hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, m, n, A, lda,
+                       std::ldexp(1.0, -e / 2), std::ldexp(1.0, -e - (-e / 2)));
+    DDC_LAUNCHED();
+    *aexp = e;
+  }
+  return DDC_OK;
+}
+
+// v[0..len) *= 2^aexp on the device (undoes check_and_scale)
+static int unscale(int len, double* v, int aexp, hipStream_t st) {
+  if (aexp == 0 || len <= 0) return DDC_OK;
+  hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, len, 1, v, len,
+                     std::ldexp(1.0, aexp / 2), std::ldexp(1.0, aexp - aexp / 2));
+  DDC_LAUNCHED();
+  return DDC_OK;
+}
+
 // Workspace of the tall (m >= n) pipeline.
 struct TallBufs {
   int lda_p, ldb;
@@ -424,34 +463,16 @@ struct Timer {
 // Results: sigma in b.sigma, U (m x n) in dU/ldu, V (n x n) in dV/ldv.
 static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu,
                        double* dV, int ldv, hipStream_t st, Timer& tm) {
-  // flag[0]: not finite; flag[2..3]: the bits of max |A|
-  unsigned long long* amax = reinterpret_cast<unsigned long long*>(b.flag + 2);
-  DDC_HIP(hipMemsetAsync(b.flag, 0, 4 * sizeof(int), st));
-  hipLaunchKernelGGL(k_check_finite, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p, b.flag,
-                     amax);
-  DDC_LAUNCHED();
-  int hflag[4] = {0, 0, 0, 0};
-  DDC_HIP(hipMemcpyAsync(hflag, b.flag, sizeof(hflag), hipMemcpyDeviceToHost, st));
-  DDC_HIP(hipStreamSynchronize(st));
-  if (hflag[0]) {
+  // A matrix whose largest entry lies outside [2^-450, 2^450] is scaled by
+  // a power of two: the values and vectors stages work with sigma^2, which
+  // would underflow or overflow.  sigma is scaled back at the end; U and V
+  // do not change.
+  bool nonfinite = false;
+  int aexp = 0;
+  DDC_TRY(check_and_scale(m, n, b.Ap, b.lda_p, b.flag, st, &nonfinite, &aexp));
+  if (nonfinite) {
     set_err("input matrix holds NaN or Inf");
     return DDC_E_NUMERIC;
-  }
-  // A matrix whose largest entry lies outside [2^-450, 2^450] is scaled by
-  // a power of two (exact) to bring it into [1/2, 1): the values and
-  // vectors stages work with sigma^2, which would underflow or overflow.
-  // sigma is scaled back at the end; U and V do not change.
-  unsigned long long abits = 0;
-  std::memcpy(&abits, hflag + 2, sizeof(abits));
-  double amx = 0.0;
-  std::memcpy(&amx, &abits, sizeof(amx));
-  int aexp = 0;
-  if (amx > 0.0) (void)std::frexp(amx, &aexp);
-  const bool scaled = amx > 0.0 && (aexp < -450 || aexp > 450);
-  if (scaled) {
-    hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, m, n, b.Ap, b.lda_p,
-                       std::ldexp(1.0, -aexp / 2), std::ldexp(1.0, -aexp - (-aexp / 2)));
-    DDC_LAUNCHED();
   }
   DDC_TRY(tm.mark());
   DDC_TRY(bidiag_run(m, n, b.Ap, b.lda_p, b.alpha, b.beta, ws, st));
@@ -461,11 +482,7 @@ static int tall_stages(int m, int n, TallBufs& b, Arena& ws, double* dU, int ldu
   DDC_TRY(twisted_run(n, b.alpha, b.beta, b.sigma, b.Vt, b.Ut, b.ldb, ws, st));
   DDC_TRY(tm.mark());
   DDC_TRY(backtrans_run(m, n, b.Ap, b.lda_p, b.Ut, b.Vt, b.ldb, dU, ldu, dV, ldv, ws, st));
-  if (scaled) {
-    hipLaunchKernelGGL(k_scale, dim3(1024), dim3(256), 0, st, n, 1, b.sigma, n,
-                       std::ldexp(1.0, aexp / 2), std::ldexp(1.0, aexp - aexp / 2));
-    DDC_LAUNCHED();
-  }
+  DDC_TRY(unscale(n, b.sigma, aexp, st));
   DDC_TRY(tm.mark());
   return DDC_OK;
 }
@@ -665,16 +682,26 @@ static int bidiag_host(int m, int n, double* A, double* alpha, double* beta) {
   plan.add<double>((size_t)lda_p * n);
   plan.add<double>(n);
   plan.add<double>(n);
+  plan.add<int>(64);
   bidiag_plan(plan, m, n, lda_p);
   Arena ws;
   DDC_TRY(arena_get(plan.bytes, ws, st));
   double* Ap = ws.take<double>((size_t)lda_p * n);
   double* da = ws.take<double>(n);
   double* db = ws.take<double>(n);
+  int* flag = ws.take<int>(64);
   DDC_HIP(hipMemsetAsync(Ap, 0, sizeof(double) * (size_t)lda_p * n, st));
   DDC_HIP(hipMemcpy2DAsync(Ap, sizeof(double) * lda_p, A, sizeof(double) * m,
                            sizeof(double) * m, n, hipMemcpyHostToDevice, st));
+  // extreme magnitudes: scaled by a power of two, alpha and beta scaled
+  // back (the reflectors left in A are dimensionless); a matrix holding NaN
+  // or Inf is bidiagonalized as it stands
+  bool nonfinite = false;
+  int aexp = 0;
+  DDC_TRY(check_and_scale(m, n, Ap, lda_p, flag, st, &nonfinite, &aexp));
   DDC_TRY(bidiag_run(m, n, Ap, lda_p, da, db, ws, st));
+  DDC_TRY(unscale(std::min(m, n), da, aexp, st));
+  DDC_TRY(unscale(nbeta, db, aexp, st));
   DDC_HIP(hipMemcpy2DAsync(A, sizeof(double) * m, Ap, sizeof(double) * lda_p,
                            sizeof(double) * m, n, hipMemcpyDeviceToHost, st));
   DDC_HIP(hipMemcpyAsync(alpha, da, sizeof(double) * std::min(m, n), hipMemcpyDeviceToHost, st));

@@ -419,6 +419,21 @@ def test_bdsvd_extreme_scales(ddcsvd, scale):
     check_svd(B0, s / scale, Y, X)
 
 
+@pytest.mark.parametrize("p", [-1000, -520, 520, 1000])
+def test_bidiag_extreme_scales(ddcsvd, p):
+    """bidiag_gpu on 2^p G: the library scales by a power of two, so the
+    reflectors left in A (dimensionless) are those of G and alpha, beta are
+    G's times 2^p (no entry of G becomes subnormal at 2^-1000)."""
+    G = gen.normal_matrix(100, 80, 3)
+    A0, al0, be0 = ddcsvd.bidiag(G)
+    gpu_case(f"bidiag_gpu 100x80 scaled by 2^{p}")
+    A, al, be = ddcsvd.bidiag(np.ldexp(G, p))
+    assert np.all(np.isfinite(A)) and np.all(np.isfinite(al)) and np.all(np.isfinite(be))
+    np.testing.assert_allclose(A, A0, rtol=0, atol=4e-16 * np.abs(A0).max())
+    np.testing.assert_allclose(np.ldexp(al, -p), al0, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(np.ldexp(be, -p), be0, rtol=1e-15, atol=0)
+
+
 def test_svd_gpu_rejects_nonfinite(ddcsvd):
     A = np.ones((4, 3))
     A[1, 1] = np.nan
