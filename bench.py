@@ -28,8 +28,9 @@ serial bidiagonalization (one core by construction) timed on the first
 steps of the same matrix and scaled by the work ratio, plus the OpenMP
 stages (values, vectors, back-transformation) from full host-path runs of
 a 1024 x 1024 matrix of the same distribution (median of three), scaled by
-their orders of growth; the 1024^2 full path, extrapolated by the survey's
-x8 .. x11.7 per doubling, brackets the bidiagonalization estimate.
+their orders of growth; the 1024^2 full path, extrapolated by x8 .. x11.7
+per doubling (O(n^3) work, and that with the cache effects of the larger
+matrix), brackets the bidiagonalization estimate.
 `cpu_lapack` is an informative, stronger CPU figure: numpy's LAPACK SVD
 (U, sigma, V) on all host threads, timed on a half-size matrix of the same
 distribution and scaled by the work ratio S(m,n)/S(m/2,n/2) = 8.
