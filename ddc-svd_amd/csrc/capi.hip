@@ -742,9 +742,9 @@ static int bdsvd_host(int n, const double* alpha, const double* beta, double* si
   double amx = 0.0;
   for (int i = 0; i < n; ++i) amx = std::fmax(amx, std::fabs(alpha[i]));
   for (int i = 0; i + 1 < n; ++i) amx = std::fmax(amx, std::fabs(beta[i]));
-  int aexp = 0;
-  const bool scaled = amx > 0.0 && std::isfinite(amx) && (std::frexp(amx, &aexp), true) &&
-                      (aexp < -450 || aexp > 450);
+  int aexp = 0;  // stays 0 for a zero or non-finite bidiagonal
+  if (amx > 0.0 && std::isfinite(amx)) (void)std::frexp(amx, &aexp);
+  const bool scaled = aexp < -450 || aexp > 450;
   std::vector<double> sa, sb;
   const double* pa = alpha;
   const double* pb = beta;
