@@ -561,7 +561,14 @@ static int svd_host(int m, int n, double* A, double* sigma, double* U, double* V
     DDC_TRY(transpose_f64(m, n, Aw, ldw, b.Ap, b.lda_p, st));
   }
   DDC_TRY(tall_stages(mt, nt, b, ws, b.U, mt, b.V, nt, st, tm));
-  if (wide) DDC_TRY(bidiag_run(m, n, Aw, ldw, wab, wab + n, ws, st));
+  if (wide) {
+    // the reflectors left in A are dimensionless: an input of extreme
+    // magnitude is scaled as in tall_stages (its alpha, beta are not kept)
+    bool nonfinite = false;
+    int aexp = 0;
+    DDC_TRY(check_and_scale(m, n, Aw, ldw, b.flag, st, &nonfinite, &aexp));
+    DDC_TRY(bidiag_run(m, n, Aw, ldw, wab, wab + n, ws, st));
+  }
   // outputs
   DDC_TRY(copy_d2h(sigma, D * nt, b.sigma, D * nt, D * nt, 1, st));
   if (!wide) {

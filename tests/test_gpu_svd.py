@@ -434,6 +434,20 @@ def test_bidiag_extreme_scales(ddcsvd, p):
     np.testing.assert_allclose(np.ldexp(be, -p), be0, rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("shape", [(100, 80), (80, 100)])
+@pytest.mark.parametrize("p", [-1000, 1000])
+def test_svd_gpu_extreme_scales_leave_scale_free_reflectors(ddcsvd, shape, p):
+    """svd_gpu on 2^p G, tall and wide: sigma is G's times 2^p and A is left
+    holding G's reflectors (dimensionless), as at unit scale."""
+    G = gen.normal_matrix(shape[0], shape[1], 3)
+    s0, _, _, A0 = ddcsvd.svd(G)
+    gpu_case(f"svd_gpu {shape[0]}x{shape[1]} scaled by 2^{p}")
+    s, U, V, A = ddcsvd.svd(np.ldexp(G, p))
+    assert np.all(np.isfinite(A)) and np.all(np.isfinite(U)) and np.all(np.isfinite(V))
+    np.testing.assert_allclose(np.ldexp(s, -p), s0, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(A, A0, rtol=0, atol=4e-16 * np.abs(A0).max())
+
+
 def test_svd_gpu_rejects_nonfinite(ddcsvd):
     A = np.ones((4, 3))
     A[1, 1] = np.nan
